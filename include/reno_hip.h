@@ -1,0 +1,150 @@
+/*
+ * reno_hip.h -- C ABI of libreno_hip.so, the MI355X (gfx950) kernels behind the
+ * one-site TDVP-PS / DMRG local update of Renormalizer.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns an
+ * int status (RENO_OK == 0, negative on error) and never throws.  Device
+ * buffers are caller-owned; the library launches asynchronously on the HIP
+ * stream it is given and allocates no device memory of its own except inside
+ * rocSOLVER (block QR/RQ), whose handle keeps a per-device pool.
+ *
+ * Tensor layouts are those of the reference (row-major numpy):
+ *   MPS site          (M_l, d, M_r)            complex
+ *   MPO site W        (D_l, d_out, d_in, D_r)  real
+ *   environment L/R   (bra, mpo, ket)          complex
+ *   bond matrix       (M_l, M_r)               complex
+ * Complex numbers are interleaved (re, im) pairs: complex128 in FP64 mode,
+ * complex64 for environments / MPO in the mixed mode (see reno_prec).
+ */
+#ifndef RENO_HIP_H
+#define RENO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define RENO_OK 0
+#define RENO_EINVAL (-1)   /* bad argument / shape mismatch (reference: AssertionError) */
+#define RENO_EHIP (-2)     /* HIP runtime error */
+#define RENO_ENOSPACE (-3) /* caller workspace too small; *needed bytes reported */
+#define RENO_ESOLVER (-4)  /* rocSOLVER failure */
+#define RENO_EQN (-5)      /* no valid quantum-number block (reference: ValueError) */
+
+/* precision modes */
+#define RENO_PREC_FP64 0  /* complex128 everywhere, FP64 MFMA */
+#define RENO_PREC_MIXED 1 /* complex64 environments + FP32 MFMA contractions,
+                             complex128 Krylov vectors, FP64 reductions */
+
+/* Effective-Hamiltonian operator, reference hop_expr (hop_expr.py:7-116).
+ * nsite = 1: out[a,d,l] = sum L[a,b,c] W[b,d,e,f] R[l,f,k] C[c,e,k]  (hop_expr.py:68-79)
+ * nsite = 0: out[a,l]   = sum L[a,b,c] R[l,b,k] S[c,k]               (hop_expr.py:57-66)
+ *            (d = 1 and W unused for nsite = 0; Dl = Dr = D)
+ * L is (Ml, Dl, Ml), R is (Mr, Dr, Mr).  In RENO_PREC_FP64 L/R are complex128
+ * and W float64; in RENO_PREC_MIXED L/R are complex64 and W float32.  Vectors
+ * (C in / out) are complex128 in both modes. */
+typedef struct {
+  int nsite;
+  int prec;
+  int Ml, d, Mr, Dl, Dr;
+  const void* L;
+  const void* W;
+  const void* R;
+} reno_hop_desc;
+
+/* One quantum-number block of svd_qn: rows `lset` x cols `rset` of the
+ * coefficient matrix (device int64 index arrays). */
+typedef struct {
+  int nrows;
+  int ncols;
+  const int64_t* rows; /* device pointer, nrows entries */
+  const int64_t* cols; /* device pointer, ncols entries */
+} reno_qn_block;
+
+/* ---- library ---------------------------------------------------------- */
+int reno_version(void);
+const char* reno_status_string(int status);
+/* last HIP / rocSOLVER error text recorded by this thread */
+const char* reno_last_error(void);
+
+/* ---- workspace sizes (bytes) ---------------------------------------- */
+size_t reno_hop_workspace_bytes(const reno_hop_desc* op);
+/* workspace for expm_krylov with room for `kcap` Krylov vectors */
+size_t reno_krylov_workspace_bytes(const reno_hop_desc* op, int kcap);
+size_t reno_env_workspace_bytes(int prec, int Ml, int d, int Mr, int Dl, int Dr);
+size_t reno_qr_workspace_bytes(int rows, int cols, int nblocks, const reno_qn_block* blocks);
+
+/* ---- hop_expr: one matvec (replaces hop_expr.py:57-91 expr(C)) ------ */
+int reno_hop_apply(const reno_hop_desc* op, const void* c_in, void* c_out,
+                   void* ws, size_t ws_bytes, void* stream);
+
+/* ---- contract_one_site (replaces lib.py:169-250) --------------------
+ * domain 0 = "L": out[f,g,h] = sum env[a,b,c] conj(ms[a,d,f]) W[b,d,e,g] ms[c,e,h]
+ *   env_in (Ml, Dl, Ml) -> env_out (Mr, Dr, Mr)
+ * domain 1 = "R": out[f,g,h] = sum conj(ms[f,d,a]) env[a,b,c] W[g,d,e,b] ms[h,e,c]
+ *   env_in (Mr, Dr, Mr) -> env_out (Ml, Dl, Ml)
+ * ms is complex128 in both modes; env/W follow `prec` as in reno_hop_desc.
+ * ms_conj may be NULL (then conj(ms) is used, lib.py:184-185); if given it is
+ * a complex128 tensor of ms's shape and conj(ms_conj) is NOT taken: the
+ * reference passes the already-conjugated bra tensor. */
+int reno_env_update(int domain, int prec, int Ml, int d, int Mr, int Dl, int Dr,
+                    const void* env_in, const void* ms, const void* ms_conj,
+                    const void* W, void* env_out, void* ws, size_t ws_bytes,
+                    void* stream);
+
+/* ---- closing contraction of two environments (the last step of
+ * Mps.expectation, mps.py:516-520, and of calc_1site_rdm, mps.py:1587-1596):
+ *   out[g] = sum_{f,h} X[f,g,h] * Y[f,h]
+ * X (F, G, H) and Y (F, H) complex128 (prec 0) or complex64 (prec 1);
+ * out (G) complex128. */
+int reno_env_close(int prec, int F, int G, int H, const void* X, const void* Y,
+                   void* out, void* stream);
+
+/* ---- expm_krylov (replaces lib/krylov/krylov.py:27-84) --------------
+ * out = expm(dt * H) v0 by Lanczos without re-orthogonalisation, with the
+ * reference's stopping rule: breakdown beta_j < 100 n eps, full-space exit,
+ * and at every even j > 3 allclose(res, new_res, rtol 1e-5, atol 1e-8).
+ * dt = dt_re + i dt_im.  v0/out are complex128 of n = Ml*d*Mr elements and
+ * may alias.  *n_matvec receives the number of matvecs (reference j+1).
+ * Returns RENO_ENOSPACE with *needed = bytes if ws holds too few Krylov
+ * vectors; nothing is written to out then and the call may be repeated. */
+int reno_expm_krylov(const reno_hop_desc* op, double dt_re, double dt_im,
+                     const void* v0, void* out, int* n_matvec, size_t* needed,
+                     void* ws, size_t ws_bytes, void* stream);
+
+/* ---- svd_qn QR / RQ branch (replaces svd_qn.py:99-240, QR=True,
+ * full_matrices=False) ------------------------------------------------
+ * coef is the complex128 (rows x cols) coefficient matrix.  For each block
+ * (in the caller's order == reference `set` iteration order, svd_qn.py:177):
+ * system 0 = "L": block = Q R   (scipy.linalg.qr, economic)
+ * system 1 = "R": block = R Q   (scipy.linalg.rq, economic)
+ * u (rows x kt) and v (cols x kt), kt = sum_b min(nrows_b, ncols_b), are
+ * zero-filled and receive each block's factors in its row/col index set at
+ * column offset sum of previous dims:
+ *   "L": u[rows_b, off:off+k_b] = Q_b,  v[cols_b, off:off+k_b] = R_b^T
+ *   "R": u[rows_b, off:off+k_b] = R_b,  v[cols_b, off:off+k_b] = Q_b^T
+ * exactly as blockappend/blockrecover (svd_qn.py:66-96) lay them out.
+ * With v_transposed != 0, v receives v^T (kt x cols) instead, the layout the
+ * sweep stores as the next site / bond matrix (mps.py:1328-1331). */
+int reno_block_qr(int system, int rows, int cols, const void* coef,
+                  int nblocks, const reno_qn_block* blocks, void* u, void* v,
+                  int v_transposed, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- tensordot (replaces matrix.py:210-211 for the 2-D contractions of
+ * the sweep: absorbing the evolved bond into the neighbouring site) ------
+ * C (M x N) = A (M x K) . B (K x N), all row-major complex128. */
+int reno_zgemm(int M, int N, int K, const void* A, const void* B, void* C,
+               void* stream);
+
+/* ---- test hook: one v_mfma 16x16x4 tile, used by the lane-layout check.
+ * a (16x4), b (4x16), c (16x16) row-major; prec selects f64 or f32. */
+int reno_probe_mfma16(int prec, const void* a, const void* b, void* c,
+                      void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RENO_HIP_H */

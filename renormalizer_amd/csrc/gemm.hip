@@ -1,0 +1,369 @@
+// Batched strided complex GEMM on CDNA4 matrix cores.
+//
+// One kernel family serves every contraction of the TDVP/DMRG local update:
+// the three stages of the effective-Hamiltonian matvec, the three stages of
+// the environment update and the bond absorption (see contract.hip).
+//
+// Tiling: a 256-thread workgroup owns a 64x64 output tile and walks K in
+// steps of 16.  Operands are staged through LDS with real and imaginary parts
+// split into separate planes (conjugation applied on the way in), so that
+// each wave feeds v_mfma_{f64,f32}_16x16x4 directly from LDS: a wave owns a
+// 32x32 sub-tile = 2x2 MFMA blocks, and a complex multiply-accumulate is four
+// real MFMAs (two when A is real, e.g. the MPO stage).  The next K-tile is
+// fetched into registers while the current one is multiplied.
+//
+// MFMA 16x16x4 lane layout (checked on the device by reno_probe_mfma16):
+//   A operand: lane l holds A[l % 16][l / 16]
+//   B operand: lane l holds B[l / 16][l % 16]
+//   C/D f32  : lane l, register i holds C[4 * (l / 16) + i][l % 16]
+//   C/D f64  : lane l, register i holds C[4 * i + l / 16][l % 16]
+#include "reno_common.h"
+
+namespace reno {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <typename R> struct Mfma;
+template <> struct Mfma<double> {
+  typedef d4_t acc;
+  __device__ static __forceinline__ acc mac(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // output row of register r in a lane whose lane/16 is q
+  __device__ static __forceinline__ int out_row(int q, int r) { return 4 * r + q; }
+};
+template <> struct Mfma<float> {
+  typedef f4_t acc;
+  __device__ static __forceinline__ acc mac(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static __forceinline__ int out_row(int q, int r) { return 4 * q + r; }
+};
+
+template <typename R>
+__device__ __forceinline__ void load_elem(const void* base, int kind, int64_t off,
+                                          R& re, R& im) {
+  switch (kind) {
+    case ST_Z128: {
+      double2 v = reinterpret_cast<const double2*>(base)[off];
+      re = (R)v.x;
+      im = (R)v.y;
+      break;
+    }
+    case ST_C64: {
+      float2 v = reinterpret_cast<const float2*>(base)[off];
+      re = (R)v.x;
+      im = (R)v.y;
+      break;
+    }
+    case ST_F64:
+      re = (R)reinterpret_cast<const double*>(base)[off];
+      im = (R)0;
+      break;
+    default:
+      re = (R)reinterpret_cast<const float*>(base)[off];
+      im = (R)0;
+      break;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void store_elem(void* base, int kind, int64_t off, R re,
+                                           R im, int beta) {
+  if (kind == ST_Z128) {
+    double2* p = reinterpret_cast<double2*>(base) + off;
+    double2 v;
+    v.x = (double)re;
+    v.y = (double)im;
+    if (beta) {
+      double2 o = *p;
+      v.x += o.x;
+      v.y += o.y;
+    }
+    *p = v;
+  } else {
+    float2* p = reinterpret_cast<float2*>(base) + off;
+    float2 v;
+    v.x = (float)re;
+    v.y = (float)im;
+    if (beta) {
+      float2 o = *p;
+      v.x += o.x;
+      v.y += o.y;
+    }
+    *p = v;
+  }
+}
+
+constexpr int kBM = 64;
+constexpr int kBN = 64;
+constexpr int kBK = 16;
+constexpr int kThreads = 256;
+constexpr int kPad = 4;
+constexpr int kPerThread = (kBM * kBK) / kThreads;  // 4 (A) and 4 (B)
+
+// AK: A is contiguous along k (sAk == 1), else along m.
+// BKC: B is contiguous along k (sBk == 1), else along n.
+// AREAL: A has no imaginary part.
+template <typename R, bool AK, bool BKC, bool AREAL>
+__global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
+  __shared__ R sA[2][kBK][kBM + kPad];
+  __shared__ R sB[2][kBK][kBN + kPad];
+  typedef typename Mfma<R>::acc acc_t;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1;
+  const int wn = wave & 1;
+
+  const int tilesN = (g.N + kBN - 1) / kBN;
+  const int tm = blockIdx.x / tilesN;
+  const int tn = blockIdx.x - tm * tilesN;
+  const int m0 = tm * kBM;
+  const int n0 = tn * kBN;
+  const int64_t bat = blockIdx.y;
+  const int64_t aoff = bat * g.sAb;
+  const int64_t boff = bat * g.sBb;
+  const int64_t coff = bat * g.sCb;
+
+  acc_t cr[2][2], ci[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cr[i][j][r] = (R)0;
+        ci[i][j][r] = (R)0;
+      }
+    }
+
+  R pa_re[kPerThread], pa_im[kPerThread], pb_re[kPerThread], pb_im[kPerThread];
+
+  auto a_coord = [&](int e, int& mm, int& kk) {
+    if (AK) {
+      kk = e % kBK;
+      mm = e / kBK;
+    } else {
+      mm = e % kBM;
+      kk = e / kBM;
+    }
+  };
+  auto b_coord = [&](int e, int& kk, int& nn) {
+    if (BKC) {
+      kk = e % kBK;
+      nn = e / kBK;
+    } else {
+      nn = e % kBN;
+      kk = e / kBN;
+    }
+  };
+
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+      const int e = tid + i * kThreads;
+      int mm, kk;
+      a_coord(e, mm, kk);
+      const int gm = m0 + mm, gk = k0 + kk;
+      R re = (R)0, im = (R)0;
+      if (gm < g.M && gk < g.K)
+        load_elem<R>(g.A, g.a_kind, aoff + (int64_t)gm * g.sAm + (int64_t)gk * g.sAk, re, im);
+      pa_re[i] = re;
+      pa_im[i] = g.conjA ? -im : im;
+      int nn;
+      b_coord(e, kk, nn);
+      const int gn = n0 + nn;
+      const int gk2 = k0 + kk;
+      re = (R)0;
+      im = (R)0;
+      if (gn < g.N && gk2 < g.K)
+        load_elem<R>(g.B, g.b_kind, boff + (int64_t)gk2 * g.sBk + (int64_t)gn * g.sBn, re, im);
+      pb_re[i] = re;
+      pb_im[i] = g.conjB ? -im : im;
+    }
+  };
+
+  auto sstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+      const int e = tid + i * kThreads;
+      int mm, kk;
+      a_coord(e, mm, kk);
+      sA[0][kk][mm] = pa_re[i];
+      if (!AREAL) sA[1][kk][mm] = pa_im[i];
+      int nn;
+      b_coord(e, kk, nn);
+      sB[0][kk][nn] = pb_re[i];
+      sB[1][kk][nn] = pb_im[i];
+    }
+  };
+
+  const int r16 = lane & 15;
+  const int kq = lane >> 4;
+
+  gload(0);
+  for (int k0 = 0; k0 < g.K; k0 += kBK) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + kBK < g.K) gload(k0 + kBK);
+#pragma unroll
+    for (int ks = 0; ks < kBK / 4; ++ks) {
+      const int kr = ks * 4 + kq;
+      R a_re[2], a_im[2], b_re[2], b_im[2];
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) {
+        a_re[mb] = sA[0][kr][wm * 32 + mb * 16 + r16];
+        a_im[mb] = AREAL ? (R)0 : sA[1][kr][wm * 32 + mb * 16 + r16];
+      }
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb) {
+        b_re[nb] = sB[0][kr][wn * 32 + nb * 16 + r16];
+        b_im[nb] = sB[1][kr][wn * 32 + nb * 16 + r16];
+      }
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+          cr[mb][nb] = Mfma<R>::mac(a_re[mb], b_re[nb], cr[mb][nb]);
+          ci[mb][nb] = Mfma<R>::mac(a_re[mb], b_im[nb], ci[mb][nb]);
+          if (!AREAL) {
+            cr[mb][nb] = Mfma<R>::mac(-a_im[mb], b_im[nb], cr[mb][nb]);
+            ci[mb][nb] = Mfma<R>::mac(a_im[mb], b_re[nb], ci[mb][nb]);
+          }
+        }
+    }
+  }
+
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 32 + mb * 16 + Mfma<R>::out_row(kq, r);
+        const int col = n0 + wn * 32 + nb * 16 + r16;
+        if (row < g.M && col < g.N)
+          store_elem<R>(g.C, g.c_kind, coff + (int64_t)row * g.sCm + (int64_t)col * g.sCn,
+                        cr[mb][nb][r], ci[mb][nb][r], g.beta);
+      }
+}
+
+template <typename R, bool AK, bool BKC>
+static void launch_t(const GemmCall& g, dim3 grid, hipStream_t s, bool areal) {
+  if (areal)
+    hipLaunchKernelGGL((k_cgemm<R, AK, BKC, true>), grid, dim3(kThreads), 0, s, g);
+  else
+    hipLaunchKernelGGL((k_cgemm<R, AK, BKC, false>), grid, dim3(kThreads), 0, s, g);
+}
+
+template <typename R>
+static void launch_r(const GemmCall& g, dim3 grid, hipStream_t s, bool ak, bool bk,
+                     bool areal) {
+  if (ak) {
+    if (bk) launch_t<R, true, true>(g, grid, s, areal);
+    else launch_t<R, true, false>(g, grid, s, areal);
+  } else {
+    if (bk) launch_t<R, false, true>(g, grid, s, areal);
+    else launch_t<R, false, false>(g, grid, s, areal);
+  }
+}
+
+int launch_gemm(const GemmCall& g, hipStream_t stream) {
+  if (g.M < 0 || g.N < 0 || g.K < 0 || g.batch < 0) {
+    set_error("gemm: negative size M=%d N=%d K=%d batch=%d", g.M, g.N, g.K, g.batch);
+    return RENO_EINVAL;
+  }
+  if (g.M == 0 || g.N == 0 || g.batch == 0) return RENO_OK;
+  if (g.sCn != 1 || (g.sAk != 1 && g.sAm != 1) || (g.sBk != 1 && g.sBn != 1)) {
+    set_error("gemm: unsupported strides sAm=%lld sAk=%lld sBk=%lld sBn=%lld sCn=%lld",
+              (long long)g.sAm, (long long)g.sAk, (long long)g.sBk, (long long)g.sBn,
+              (long long)g.sCn);
+    return RENO_EINVAL;
+  }
+  if (g.c_kind != ST_Z128 && g.c_kind != ST_C64) {
+    set_error("gemm: output must be complex");
+    return RENO_EINVAL;
+  }
+  if (g.batch > 65535) {
+    set_error("gemm: batch %d exceeds grid limit", g.batch);
+    return RENO_EINVAL;
+  }
+  const bool areal = (g.a_kind == ST_F64 || g.a_kind == ST_F32);
+  if (g.b_kind == ST_F64 || g.b_kind == ST_F32) {
+    set_error("gemm: B must be complex");
+    return RENO_EINVAL;
+  }
+  const int64_t tiles = (int64_t)((g.M + kBM - 1) / kBM) * ((g.N + kBN - 1) / kBN);
+  if (tiles > 0x7fffffff) {
+    set_error("gemm: too many tiles");
+    return RENO_EINVAL;
+  }
+  dim3 grid((unsigned)tiles, (unsigned)g.batch);
+  const bool ak = (g.sAk == 1);
+  const bool bk = (g.sBk == 1) && !(g.sBn == 1 && g.K == 1);
+  if (g.K == 0) {
+    // C = 0 (beta = 0) or unchanged (beta = 1): run the kernel with K = 0,
+    // which stores the zero accumulators.
+  }
+  if (g.prec == RENO_PREC_FP64)
+    launch_r<double>(g, grid, stream, ak, bk, areal);
+  else
+    launch_r<float>(g, grid, stream, ak, bk, areal);
+  RENO_CHECK_LAUNCH();
+  return RENO_OK;
+}
+
+// ---- lane-layout probe: one MFMA 16x16x4 on row-major a (16x4), b (4x16);
+// writes the raw accumulator registers c[lane*4 + r] -------------------------
+template <typename R>
+__global__ void k_probe_mfma16(const R* a, const R* b, R* c) {
+  const int lane = threadIdx.x;
+  typename Mfma<R>::acc acc;
+  for (int r = 0; r < 4; ++r) acc[r] = (R)0;
+  const R av = a[(lane % 16) * 4 + lane / 16];
+  const R bv = b[(lane / 16) * 16 + lane % 16];
+  acc = Mfma<R>::mac(av, bv, acc);
+  for (int r = 0; r < 4; ++r) c[lane * 4 + r] = acc[r];
+}
+
+}  // namespace reno
+
+extern "C" int reno_probe_mfma16(int prec, const void* a, const void* b, void* c,
+                                 void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == RENO_PREC_FP64)
+    hipLaunchKernelGGL(reno::k_probe_mfma16<double>, dim3(1), dim3(64), 0, s,
+                       (const double*)a, (const double*)b, (double*)c);
+  else
+    hipLaunchKernelGGL(reno::k_probe_mfma16<float>, dim3(1), dim3(64), 0, s,
+                       (const float*)a, (const float*)b, (float*)c);
+  RENO_CHECK_LAUNCH();
+  return RENO_OK;
+}
+
+extern "C" int reno_zgemm(int M, int N, int K, const void* A, const void* B, void* C,
+                          void* stream) {
+  reno::GemmCall g{};
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.batch = 1;
+  g.prec = RENO_PREC_FP64;
+  g.A = A;
+  g.a_kind = reno::ST_Z128;
+  g.sAm = K;
+  g.sAk = 1;
+  g.B = B;
+  g.b_kind = reno::ST_Z128;
+  g.sBk = N;
+  g.sBn = 1;
+  g.C = C;
+  g.c_kind = reno::ST_Z128;
+  g.sCm = N;
+  g.sCn = 1;
+  return reno::launch_gemm(g, (hipStream_t)stream);
+}

@@ -1,0 +1,85 @@
+// Shared definitions for the libreno_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/reno_hip.h"
+
+namespace reno {
+
+// Per-thread text of the last runtime error, read by reno_last_error().
+void set_error(const char* fmt, ...);
+
+#define RENO_HIP_CHECK(expr)                                                   \
+  do {                                                                         \
+    hipError_t _e = (expr);                                                    \
+    if (_e != hipSuccess) {                                                    \
+      ::reno::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #expr,           \
+                        hipGetErrorString(_e));                                \
+      return RENO_EHIP;                                                        \
+    }                                                                          \
+  } while (0)
+
+#define RENO_CHECK_LAUNCH()                                                    \
+  do {                                                                         \
+    hipError_t _e = hipGetLastError();                                         \
+    if (_e != hipSuccess) {                                                    \
+      ::reno::set_error("%s:%d launch -> %s", __FILE__, __LINE__,              \
+                        hipGetErrorString(_e));                                \
+      return RENO_EHIP;                                                        \
+    }                                                                          \
+  } while (0)
+
+#define RENO_TRY(expr)                                                         \
+  do {                                                                         \
+    int _s = (expr);                                                           \
+    if (_s != RENO_OK) return _s;                                              \
+  } while (0)
+
+// Interleaved complex storage types (match torch complex128 / complex64).
+struct __align__(16) z128 {
+  double re, im;
+};
+struct __align__(8) c64 {
+  float re, im;
+};
+
+template <typename S> struct real_of;
+template <> struct real_of<z128> { typedef double type; };
+template <> struct real_of<c64> { typedef float type; };
+
+// Generic batched strided complex GEMM:
+//   C[b](m,n) (+)= sum_k opA(A[b])(m,k) * opB(B[b])(k,n)
+// Element (m,k) of A lives at A + b*sAb + m*sAm + k*sAk (units: elements of
+// the storage type).  Either sAk == 1 or sAm == 1 (same for B with sBk/sBn);
+// C must have sCn == 1.  a_real: A is a real matrix (storage type float or
+// double, matching the compute precision) and the imaginary MFMA products are
+// skipped.  conj flags conjugate A / B on load.  beta 0 overwrites, 1 adds.
+enum StorageKind { ST_Z128 = 0, ST_C64 = 1, ST_F64 = 2, ST_F32 = 3 };
+
+struct GemmCall {
+  int M, N, K, batch;
+  int prec;  // RENO_PREC_FP64 -> FP64 MFMA, RENO_PREC_MIXED -> FP32 MFMA
+  const void* A;
+  int a_kind;
+  int64_t sAm, sAk, sAb;
+  int conjA;
+  const void* B;
+  int b_kind;
+  int64_t sBk, sBn, sBb;
+  int conjB;
+  void* C;
+  int c_kind;
+  int64_t sCm, sCn, sCb;
+  int beta;
+};
+
+int launch_gemm(const GemmCall& g, hipStream_t stream);
+
+inline size_t align_up(size_t x, size_t a = 256) { return (x + a - 1) / a * a; }
+
+inline size_t elem_bytes(int prec) { return prec == RENO_PREC_FP64 ? 16 : 8; }
+
+}  // namespace reno

@@ -1,0 +1,3 @@
+from renormalizer_amd.mps.mpo import Mpo
+from renormalizer_amd.mps.mps import Mps, expand_bond_dimension
+from renormalizer_amd.mps.lib import Environ, contract_one_site

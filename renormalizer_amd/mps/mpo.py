@@ -1,0 +1,158 @@
+"""Matrix product operator (reference mps/mpo.py:250-494).
+
+``Mpo(model, terms=None, offset=Quantity(0))`` builds the operator from the
+model's sum-of-products terms with the QR-based symbolic construction
+(symbolic_mpo.py).  Site tensors are float64 (complex when an operator
+matrix is complex), layout (D_l, d_out, d_in, D_r), kept on the host; the
+device copies used by the TDVP-PS kernels (float64 for the FP64 mode,
+float32 for the mixed mode) are made on first use and cached.
+"""
+from typing import List, Union
+
+import numpy as np
+import torch
+
+from renormalizer_amd.model import Model, Op
+from renormalizer_amd.mps.mp import MatrixProduct
+from renormalizer_amd.mps.svd_qn import add_outer
+from renormalizer_amd.mps.symbolic_mpo import (construct_symbolic_mpo, symbolic_mo_to_numeric_mo,
+                                               terms_to_table)
+from renormalizer_amd.utils import Quantity
+
+
+class Mpo(MatrixProduct):
+    @classmethod
+    def identity(cls, model: Model):
+        mpo = cls()
+        mpo.model = model
+        for p in model.pbond_list:
+            mpo.append(np.eye(p).reshape(1, p, p, 1))
+        mpo.build_empty_qn()
+        return mpo
+
+    @classmethod
+    def onsite(cls, model: Model, opera, dipole=False, dof_set=None):
+        if dof_set is None:
+            if model.n_edofs == 0:
+                raise ValueError("No electronic DoF present in the model.")
+            dof_set = model.e_dofs
+        ops = [Op(opera, idx, model.dipole[idx] if dipole else 1.0) for idx in dof_set]
+        return cls(model, ops)
+
+    def __init__(self, model: Model = None, terms: Union[Op, List[Op]] = None,
+                 offset: Quantity = Quantity(0), algo="qr"):
+        super().__init__()
+        self._device_cache = {}
+        if model is None:
+            return
+        if algo != "qr":
+            raise NotImplementedError("only the QR construction (the reference default) is provided")
+        if not isinstance(offset, Quantity):
+            raise ValueError(f"offset must be Quantity object. Got {offset} of {type(offset)}.")
+        self.offset = offset.as_au()
+        if terms is None:
+            terms = model.ham_terms
+        elif isinstance(terms, Op):
+            terms = [terms]
+        if len(terms) == 0:
+            raise ValueError("Terms contain nothing.")
+        terms = model.check_operator_terms(terms)
+        if len(terms) == 0:
+            raise ValueError("Terms all have factor 0.")
+        table, primary_ops, factor = terms_to_table(model, terms, -self.offset)
+        self.dtype = factor.dtype
+        self.symbolic_mpo, self.qn, self.qntot, self.qnidx, self.symbolic_out_ops_list = \
+            construct_symbolic_mpo(table, primary_ops, factor)
+        self.primary_ops = primary_ops
+        self.model = model
+        self.to_right = False
+        for impo, mo in enumerate(self.symbolic_mpo):
+            self.append(symbolic_mo_to_numeric_mo(model.basis[impo], mo, self.dtype))
+
+    def _invalidate(self):
+        self._device_cache = {}
+
+    def _get_sigmaqn(self, idx):
+        up = self.model.basis[idx].sigmaqn
+        return add_outer(up, -up)
+
+    @property
+    def is_mps(self):
+        return False
+
+    @property
+    def is_mpo(self):
+        return True
+
+    def metacopy(self):
+        new = super().metacopy()
+        new._device_cache = {}
+        for attr in ("offset", "symbolic_out_ops_list", "primary_ops"):
+            if hasattr(self, attr):
+                setattr(new, attr, getattr(self, attr))
+        return new
+
+    def device_tensors(self, prec: int, device=None) -> List[torch.Tensor]:
+        """Site tensors on the device in the dtype of precision mode ``prec``
+        (0: float64, 1: float32)."""
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        key = (prec, str(device))
+        cached = getattr(self, "_device_cache", None)
+        if cached is None:
+            self._device_cache = cached = {}
+        if key not in cached:
+            if np.iscomplexobj(np.asarray(self._mp[0])) or any(np.iscomplexobj(m) for m in self._mp):
+                raise NotImplementedError("complex MPO tensors are not supported by the device kernels")
+            dt = torch.float64 if prec == 0 else torch.float32
+            cached[key] = [torch.from_numpy(np.ascontiguousarray(m, dtype=np.float64)).to(device=device, dtype=dt)
+                           for m in self._mp]
+        return cached[key]
+
+    def apply(self, mp: MatrixProduct, canonicalise: bool = False) -> MatrixProduct:
+        """MPO x MPS on host tensors (initial-state preparation; reference
+        mpo.py:331-389)."""
+        mp._require_host("Mpo.apply")
+        assert self.site_num == mp.site_num
+        new = mp.copy()
+        if np.iscomplexobj(self._mp[0]) and not new.is_complex:
+            new.to_complex(inplace=True)
+        if mp.is_mps:
+            for i, (w, m) in enumerate(zip(self._mp, mp._mp)):
+                assert w.shape[2] == m.shape[1]
+                t = np.moveaxis(np.tensordot(w, m, axes=([2], [1])), 3, 1)
+                new._mp[i] = np.ascontiguousarray(
+                    t.reshape(w.shape[0] * m.shape[0], w.shape[1], w.shape[-1] * m.shape[-1]))
+        else:
+            for i, (w, m) in enumerate(zip(self._mp, mp._mp)):
+                t = np.moveaxis(np.tensordot(w, m, axes=([2], [1])), [-3, -2], [1, 3])
+                new._mp[i] = np.ascontiguousarray(
+                    t.reshape(w.shape[0] * m.shape[0], w.shape[1], m.shape[2], w.shape[-1] * m.shape[-1]))
+        orig = new.qnidx
+        new.move_qnidx(self.qnidx)
+        new.qn = [add_outer(np.array(qo), np.array(qm)).reshape(-1, np.array(qo).shape[1])
+                  for qo, qm in zip(self.qn, new.qn)]
+        new.qntot = new.qntot + self.qntot
+        new.move_qnidx(orig)
+        if canonicalise:
+            new.canonicalise()
+        return new
+
+    def __matmul__(self, other):
+        return self.apply(other)
+
+    def todense(self):
+        """Dense matrix of the operator (small systems only)."""
+        dim = int(np.prod(self.pbond_list))
+        if dim > 20000:
+            raise ValueError("operator too large")
+        res = np.ones((1, 1, 1, 1))
+        for mt in self._mp:
+            d1 = res.shape[1] * mt.shape[1]
+            d2 = res.shape[2] * mt.shape[2]
+            res = np.tensordot(res, mt, axes=1).transpose((0, 1, 3, 2, 4, 5)).reshape(1, d1, d2, mt.shape[-1])
+        return res[0, :, :, 0]
+
+    def is_hermitian(self):
+        full = self.todense()
+        return np.allclose(full.conj().T, full, atol=1e-7)

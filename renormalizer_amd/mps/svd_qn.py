@@ -1,0 +1,123 @@
+"""Quantum-number blocked decompositions (reference mps/svd_qn.py:99-317).
+
+* ``qn_blocks`` - the host-side block bookkeeping: the block order is the
+  iteration order of ``set(tuple(t) for t in localqnl)`` exactly as in the
+  reference (svd_qn.py:177), so block layout and new bond quantum numbers are
+  bit-identical.
+* ``svd_qn_device`` - the QR/RQ branch (QR=True, economic) on the device
+  through ``reno_block_qr`` (rocSOLVER Householder factorisations).
+* ``svd_qn_host`` - the full routine (SVD and QR branches) in host numpy,
+  used only while preparing initial states (canonicalise / compress of the
+  starting MPS) before the state is uploaded.
+"""
+import numpy as np
+import scipy.linalg
+
+from renormalizer_amd import native
+
+
+def add_outer(a: np.ndarray, b: np.ndarray):
+    """np.add.outer over all but the last (quantum-number) axis."""
+    assert a.shape[-1] == b.shape[-1]
+    out = np.array([np.add.outer(a[..., i], b[..., i]) for i in range(a.shape[-1])])
+    return out.transpose(list(range(1, out.ndim)) + [0])
+
+
+def get_qn_mask(qnmat: np.ndarray, qntot):
+    return np.all(qnmat == np.array(qntot), axis=-1)
+
+
+def qn_blocks(qnbigl, qnbigr, qntot):
+    """[(nl, nr, lset, rset)] in the reference block order."""
+    qn_size = len(qntot)
+    localqnl = np.asarray(qnbigl).reshape(-1, qn_size)
+    localqnr = np.asarray(qnbigr).reshape(-1, qn_size)
+    blocks = []
+    for nl in set([tuple(t) for t in localqnl]):
+        nr = qntot - nl
+        rset = np.where(get_qn_mask(localqnr, nr))[0]
+        if len(rset) == 0:
+            continue
+        lset = np.where(get_qn_mask(localqnl, nl))[0]
+        blocks.append((nl, nr, lset, rset))
+    return blocks
+
+
+def svd_qn_device(coef, qnbigl, qnbigr, qntot, system):
+    """QR ("L") / RQ ("R") of the device coefficient tensor.
+
+    Returns (u, qnl, vt, qnr): u (rows x k), vt (k x cols) complex128 device
+    tensors with coef = u @ vt, and the new bond quantum numbers as lists of
+    tuples (the reference's ``new_qnl``/``new_qnr``).
+    """
+    rows = int(np.prod(qnbigl.shape[:-1]))
+    cols = int(np.prod(qnbigr.shape[:-1]))
+    coef2 = coef.reshape(rows, cols)
+    blocks = qn_blocks(qnbigl, qnbigr, qntot)
+    if not blocks:
+        raise ValueError("Invalid quantum number")
+    u, vt = native.block_qr(system, coef2, [(b[2], b[3]) for b in blocks], v_transposed=True)
+    qnl, qnr = [], []
+    for nl, nr, lset, rset in blocks:
+        k = min(len(lset), len(rset))
+        qnl += [nl] * k
+        qnr += [nr] * k
+    return u, qnl, vt, qnr
+
+
+def optimized_svd(a, full_matrices):
+    try:
+        return scipy.linalg.svd(a, full_matrices=full_matrices, lapack_driver="gesdd")
+    except scipy.linalg.LinAlgError:
+        return scipy.linalg.svd(a, full_matrices=full_matrices, lapack_driver="gesvd")
+
+
+def svd_qn_host(coef_array, qnbigl, qnbigr, qntot, QR=False, system=None, full_matrices=True):
+    """Host version of svd_qn (SVD and QR branches) for initial-state setup.
+    ``full_matrices=True`` returns the complete bases of every block."""
+    coef_matrix = coef_array.reshape((int(np.prod(qnbigl.shape[:-1])), int(np.prod(qnbigr.shape[:-1]))))
+    u_l, u0_l, v_l, v0_l, s_l, su0_l, sv0_l = [], [], [], [], [], [], []
+    qnl, qnl0, qnr, qnr0 = [], [], [], []
+    for nl, nr, lset, rset in qn_blocks(qnbigl, qnbigr, qntot):
+        block = coef_matrix[np.ix_(lset, rset)]
+        dim = min(block.shape)
+        if QR:
+            mode = "full" if full_matrices else "economic"
+            if system == "R":
+                bu, bvt = scipy.linalg.rq(block, mode=mode)
+            elif system == "L":
+                bu, bvt = scipy.linalg.qr(block, mode=mode)
+            else:
+                raise AssertionError("system")
+        else:
+            bu, bs, bvt = optimized_svd(block, full_matrices)
+            s_l.append(bs)
+        for src, lst, lst0, ql, ql0, sl0, idx, n, size in (
+                (bu, u_l, u0_l, qnl, qnl0, su0_l, lset, nl, coef_matrix.shape[0]),
+                (bvt.T, v_l, v0_l, qnr, qnr0, sv0_l, rset, nr, coef_matrix.shape[1])):
+            rec = np.zeros((size, dim), dtype=src.dtype)
+            rec[idx, :] = src[:, :dim]
+            lst.append(rec)
+            ql += [n] * dim
+            if full_matrices:
+                rec0 = np.zeros((size, src.shape[1] - dim), dtype=src.dtype)
+                rec0[idx, :] = src[:, dim:]
+                lst0.append(rec0)
+                ql0 += [n] * (src.shape[1] - dim)
+                sl0.append(np.zeros(src.shape[1] - dim))
+    if len(u_l) + len(u0_l) == 0 or len(v_l) + len(v0_l) == 0:
+        raise ValueError("Invalid quantum number")
+    u = np.concatenate(u_l + u0_l, axis=1)
+    v = np.concatenate(v_l + v0_l, axis=1)
+    new_qnl, new_qnr = qnl + qnl0, qnr + qnr0
+    if QR:
+        return u, new_qnl, v, new_qnr
+    su = np.concatenate(s_l + su0_l)
+    sv = np.concatenate(s_l + sv0_l)
+    if not full_matrices:
+        order = np.argsort(su)[::-1]
+        u, v = u[:, order], v[:, order]
+        su = sv = su[order]
+        new_qnl = np.array(new_qnl)[order].tolist()
+        new_qnr = np.array(new_qnr)[order].tolist()
+    return u, su, new_qnl, v, sv, new_qnr

@@ -1,0 +1,356 @@
+"""ctypes binding of ``libreno_hip.so`` (declared in ``include/reno_hip.h``).
+
+This module is the only place that talks to the C ABI.  Tensors are torch
+tensors on the current HIP device; raw device pointers and the current
+stream handle are handed to the library, which launches asynchronously on
+that stream.  There is no fallback: if the library is missing, or no GPU is
+present, every compute entry point raises.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libreno_hip.so")
+
+RENO_OK = 0
+RENO_EINVAL = -1
+RENO_EHIP = -2
+RENO_ENOSPACE = -3
+RENO_ESOLVER = -4
+RENO_EQN = -5
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+# exported symbols of include/reno_hip.h
+EXPORTS = (
+    "reno_version", "reno_status_string", "reno_last_error",
+    "reno_hop_workspace_bytes", "reno_krylov_workspace_bytes",
+    "reno_env_workspace_bytes", "reno_qr_workspace_bytes",
+    "reno_hop_apply", "reno_env_update", "reno_env_close", "reno_expm_krylov",
+    "reno_block_qr", "reno_zgemm", "reno_probe_mfma16",
+)
+
+
+class RenoNativeError(RuntimeError):
+    """Raised when the native library is unavailable or reports a failure."""
+
+
+class HopDesc(ctypes.Structure):
+    _fields_ = [("nsite", ctypes.c_int), ("prec", ctypes.c_int),
+                ("Ml", ctypes.c_int), ("d", ctypes.c_int), ("Mr", ctypes.c_int),
+                ("Dl", ctypes.c_int), ("Dr", ctypes.c_int),
+                ("L", ctypes.c_void_p), ("W", ctypes.c_void_p), ("R", ctypes.c_void_p)]
+
+
+class QnBlock(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int), ("ncols", ctypes.c_int),
+                ("rows", ctypes.c_void_p), ("cols", ctypes.c_void_p)]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the shared library and declare the C signatures (no GPU needed)."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise RenoNativeError(
+                f"{path} not found: build it with `python -m renormalizer_amd.csrc.build`")
+        lib = ctypes.CDLL(path)
+        c_int, c_size, vp, c_double = ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_double
+        lib.reno_version.restype = c_int
+        lib.reno_status_string.restype = ctypes.c_char_p
+        lib.reno_status_string.argtypes = [c_int]
+        lib.reno_last_error.restype = ctypes.c_char_p
+        lib.reno_hop_workspace_bytes.restype = c_size
+        lib.reno_hop_workspace_bytes.argtypes = [ctypes.POINTER(HopDesc)]
+        lib.reno_krylov_workspace_bytes.restype = c_size
+        lib.reno_krylov_workspace_bytes.argtypes = [ctypes.POINTER(HopDesc), c_int]
+        lib.reno_env_workspace_bytes.restype = c_size
+        lib.reno_env_workspace_bytes.argtypes = [c_int] * 6
+        lib.reno_qr_workspace_bytes.restype = c_size
+        lib.reno_qr_workspace_bytes.argtypes = [c_int, c_int, c_int, ctypes.POINTER(QnBlock)]
+        lib.reno_hop_apply.restype = c_int
+        lib.reno_hop_apply.argtypes = [ctypes.POINTER(HopDesc), vp, vp, vp, c_size, vp]
+        lib.reno_env_update.restype = c_int
+        lib.reno_env_update.argtypes = [c_int] * 7 + [vp, vp, vp, vp, vp, vp, c_size, vp]
+        lib.reno_expm_krylov.restype = c_int
+        lib.reno_expm_krylov.argtypes = [ctypes.POINTER(HopDesc), c_double, c_double, vp, vp,
+                                         ctypes.POINTER(c_int), ctypes.POINTER(c_size),
+                                         vp, c_size, vp]
+        lib.reno_block_qr.restype = c_int
+        lib.reno_block_qr.argtypes = [c_int, c_int, c_int, vp, c_int, ctypes.POINTER(QnBlock),
+                                      vp, vp, c_int, vp, c_size, vp]
+        lib.reno_env_close.restype = c_int
+        lib.reno_env_close.argtypes = [c_int, c_int, c_int, c_int, vp, vp, vp, vp]
+        lib.reno_zgemm.restype = c_int
+        lib.reno_zgemm.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp]
+        lib.reno_probe_mfma16.restype = c_int
+        lib.reno_probe_mfma16.argtypes = [c_int, vp, vp, vp, vp]
+        _lib = lib
+        return lib
+
+
+def lib():
+    return load_library()
+
+
+def _raise(status: int, what: str):
+    l = lib()
+    msg = f"{what}: {l.reno_status_string(status).decode()} ({l.reno_last_error().decode()})"
+    if status == RENO_EQN:
+        raise ValueError("Invalid quantum number. " + msg)
+    if status == RENO_EINVAL:
+        raise ValueError(msg)
+    raise RenoNativeError(msg)
+
+
+def check(status: int, what: str):
+    if status != RENO_OK:
+        _raise(status, what)
+
+
+def _require_device(*tensors):
+    for t in tensors:
+        if t is not None and not t.is_cuda:
+            raise RenoNativeError("libreno_hip operates on device tensors only; got a CPU tensor")
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class _Workspace:
+    """Grow-only device scratch shared by consecutive calls on one stream."""
+
+    def __init__(self):
+        self._buf = {}
+
+    def get(self, nbytes: int, device):
+        key = (device.index if device.index is not None else torch.cuda.current_device())
+        buf = self._buf.get(key)
+        if buf is None or buf.numel() < nbytes:
+            size = max(int(nbytes), 1 << 20)
+            if buf is not None:
+                size = max(size, int(buf.numel() * 1.25))
+            buf = torch.empty(size, dtype=torch.uint8, device=device)
+            self._buf[key] = buf
+        return buf
+
+
+workspace = _Workspace()
+
+
+def prec_dtypes(prec: int):
+    """(environment dtype, MPO dtype) for a precision mode."""
+    if prec == PREC_FP64:
+        return torch.complex128, torch.float64
+    return torch.complex64, torch.float32
+
+
+def _hop_desc(nsite, prec, L, W, R, Ml, d, Mr, Dl, Dr):
+    return HopDesc(nsite, prec, Ml, d, Mr, Dl, Dr, _ptr(L).value, _ptr(W).value if W is not None else None,
+                   _ptr(R).value)
+
+
+def _check_env_dtype(prec, *ts):
+    cdt, rdt = prec_dtypes(prec)
+    for t in ts:
+        if t is None:
+            continue
+        want = rdt if not t.is_complex() else cdt
+        if t.dtype != want:
+            raise ValueError(f"tensor dtype {t.dtype} does not match precision mode {prec} ({want})")
+        if not t.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+
+
+def hop_apply(L, W, R, c, prec=PREC_FP64):
+    """One effective-Hamiltonian matvec (reference hop_expr(...)(c)).
+
+    L (Ml, Dl, Ml), R (Mr, Dr, Mr), W (Dl, d, d, Dr) for a site vector
+    c (Ml, d, Mr); W is None for the bond (0-site) operator c (Ml, Mr).
+    """
+    _require_device(L, R, c, W)
+    _check_env_dtype(prec, L, R, W)
+    if c.dtype != torch.complex128 or not c.is_contiguous():
+        raise ValueError("vector must be contiguous complex128")
+    if W is None:
+        Ml, Mr = c.shape
+        D = L.shape[1]
+        assert L.shape == (Ml, D, Ml) and R.shape == (Mr, D, Mr), "hop0 shape mismatch"
+        desc = _hop_desc(0, prec, L, None, R, Ml, 1, Mr, D, D)
+    else:
+        Ml, d, Mr = c.shape
+        Dl, Dr = W.shape[0], W.shape[3]
+        assert L.shape == (Ml, Dl, Ml) and R.shape == (Mr, Dr, Mr), "hop1 shape mismatch"
+        assert W.shape == (Dl, d, d, Dr), "hop1 MPO shape mismatch"
+        desc = _hop_desc(1, prec, L, W, R, Ml, d, Mr, Dl, Dr)
+    l = lib()
+    nbytes = l.reno_hop_workspace_bytes(ctypes.byref(desc))
+    ws = workspace.get(nbytes, c.device)
+    out = torch.empty_like(c)
+    check(l.reno_hop_apply(ctypes.byref(desc), _ptr(c), _ptr(out), _ptr(ws), nbytes, _stream()),
+          "reno_hop_apply")
+    return out
+
+
+def env_update(env, ms, W, domain, ms_conj=None, prec=PREC_FP64):
+    """contract_one_site (reference lib.py:169-250) on the device."""
+    _require_device(env, ms, W, ms_conj)
+    _check_env_dtype(prec, env, W)
+    if ms.dtype != torch.complex128 or not ms.is_contiguous():
+        raise ValueError("site tensor must be contiguous complex128")
+    Ml, d, Mr = ms.shape
+    Dl, Dr = W.shape[0], W.shape[3]
+    assert W.shape == (Dl, d, d, Dr)
+    cdt, _ = prec_dtypes(prec)
+    if domain == "L":
+        assert env.shape == (Ml, Dl, Ml), f"L environment shape {tuple(env.shape)} vs site {tuple(ms.shape)}"
+        out = torch.empty((Mr, Dr, Mr), dtype=cdt, device=ms.device)
+        dom = 0
+    elif domain == "R":
+        assert env.shape == (Mr, Dr, Mr), f"R environment shape {tuple(env.shape)} vs site {tuple(ms.shape)}"
+        out = torch.empty((Ml, Dl, Ml), dtype=cdt, device=ms.device)
+        dom = 1
+    else:
+        raise ValueError(f"domain {domain}")
+    if ms_conj is not None:
+        assert ms_conj.shape == ms.shape and ms_conj.dtype == torch.complex128
+    l = lib()
+    nbytes = l.reno_env_workspace_bytes(prec, Ml, d, Mr, Dl, Dr)
+    ws = workspace.get(nbytes, ms.device)
+    check(l.reno_env_update(dom, prec, Ml, d, Mr, Dl, Dr, _ptr(env), _ptr(ms), _ptr(ms_conj),
+                            _ptr(W), _ptr(out), _ptr(ws), nbytes, _stream()),
+          "reno_env_update")
+    return out
+
+
+_KRYLOV_KCAP = 20
+
+
+def expm_krylov(L, W, R, v, dt, prec=PREC_FP64):
+    """expm(dt * H_eff) v by Lanczos (reference lib/krylov/krylov.py:27-84).
+
+    Returns (vector, number of matvecs) like the reference.
+    """
+    _require_device(L, R, v, W)
+    _check_env_dtype(prec, L, R, W)
+    if v.dtype != torch.complex128 or not v.is_contiguous():
+        raise ValueError("vector must be contiguous complex128")
+    if W is None:
+        Ml, Mr = v.shape
+        D = L.shape[1]
+        assert L.shape == (Ml, D, Ml) and R.shape == (Mr, D, Mr), "hop0 shape mismatch"
+        desc = _hop_desc(0, prec, L, None, R, Ml, 1, Mr, D, D)
+    else:
+        Ml, d, Mr = v.shape
+        Dl, Dr = W.shape[0], W.shape[3]
+        assert L.shape == (Ml, Dl, Ml) and R.shape == (Mr, Dr, Mr), "hop1 shape mismatch"
+        desc = _hop_desc(1, prec, L, W, R, Ml, d, Mr, Dl, Dr)
+    dt = complex(dt)
+    l = lib()
+    kcap = _KRYLOV_KCAP
+    out = torch.empty_like(v)
+    nmv = ctypes.c_int(0)
+    needed = ctypes.c_size_t(0)
+    while True:
+        nbytes = l.reno_krylov_workspace_bytes(ctypes.byref(desc), kcap)
+        ws = workspace.get(nbytes, v.device)
+        st = l.reno_expm_krylov(ctypes.byref(desc), dt.real, dt.imag, _ptr(v), _ptr(out),
+                                ctypes.byref(nmv), ctypes.byref(needed), _ptr(ws), ws.numel(),
+                                _stream())
+        if st == RENO_ENOSPACE:
+            kcap *= 2
+            continue
+        check(st, "reno_expm_krylov")
+        return out, nmv.value
+
+
+def block_qr(system, coef, blocks, v_transposed=False):
+    """QN-blocked QR ("L") / RQ ("R") of a (rows x cols) complex128 matrix.
+
+    ``blocks`` is a list of (row_index_array, col_index_array) numpy int64
+    arrays in the reference's block order.  Returns (u, v) as in
+    svd_qn(..., QR=True, full_matrices=False); with ``v_transposed`` the
+    second factor is returned as v^T (k x cols).
+    """
+    _require_device(coef)
+    if coef.dtype != torch.complex128 or coef.dim() != 2 or not coef.is_contiguous():
+        raise ValueError("coefficient matrix must be a contiguous 2-D complex128 tensor")
+    rows, cols = coef.shape
+    if len(blocks) == 0:
+        raise ValueError("Invalid quantum number")
+    dev = coef.device
+    idx = []
+    cblocks = (QnBlock * len(blocks))()
+    kt = 0
+    for i, (r, c) in enumerate(blocks):
+        rt = torch.as_tensor(np.ascontiguousarray(r, dtype=np.int64)).to(dev, non_blocking=False)
+        ct = torch.as_tensor(np.ascontiguousarray(c, dtype=np.int64)).to(dev, non_blocking=False)
+        idx.append((rt, ct))
+        cblocks[i] = QnBlock(len(r), len(c), rt.data_ptr(), ct.data_ptr())
+        kt += min(len(r), len(c))
+    u = torch.empty((rows, kt), dtype=torch.complex128, device=dev)
+    v = torch.empty((kt, cols) if v_transposed else (cols, kt), dtype=torch.complex128, device=dev)
+    l = lib()
+    nbytes = l.reno_qr_workspace_bytes(rows, cols, len(blocks), cblocks)
+    ws = workspace.get(nbytes, dev)
+    sysc = {"L": 0, "R": 1}[system]
+    check(l.reno_block_qr(sysc, rows, cols, _ptr(coef), len(blocks), cblocks, _ptr(u), _ptr(v),
+                          int(bool(v_transposed)), _ptr(ws), nbytes, _stream()), "reno_block_qr")
+    # keep the index tensors alive until the kernels that read them have run
+    torch.cuda.current_stream().synchronize()
+    del idx
+    return u, v
+
+
+def env_close(X, Y):
+    """out[g] = sum_{f,h} X[f,g,h] Y[f,h] for X (F,G,H), Y (F,H) or (F,1,H)."""
+    _require_device(X, Y)
+    if X.dtype != Y.dtype or X.dtype not in (torch.complex128, torch.complex64):
+        raise ValueError("env_close needs complex operands of one dtype")
+    X = X.contiguous()
+    Y = Y.contiguous()
+    F, G, H = X.shape
+    assert Y.numel() == F * H and Y.shape[0] == F and Y.shape[-1] == H, "env_close shape mismatch"
+    prec = PREC_FP64 if X.dtype == torch.complex128 else PREC_MIXED
+    out = torch.empty(G, dtype=torch.complex128, device=X.device)
+    check(lib().reno_env_close(prec, F, G, H, _ptr(X), _ptr(Y), _ptr(out), _stream()), "reno_env_close")
+    return out
+
+
+def zgemm(A, B):
+    """C = A @ B for 2-D contiguous complex128 device matrices."""
+    _require_device(A, B)
+    if A.dtype != torch.complex128 or B.dtype != torch.complex128:
+        raise ValueError("zgemm needs complex128 operands")
+    A = A.contiguous()
+    B = B.contiguous()
+    M, K = A.shape
+    K2, N = B.shape
+    assert K == K2, "zgemm inner dimension mismatch"
+    C = torch.empty((M, N), dtype=torch.complex128, device=A.device)
+    check(lib().reno_zgemm(M, N, K, _ptr(A), _ptr(B), _ptr(C), _stream()), "reno_zgemm")
+    return C
+
+
+def probe_mfma16(a, b, prec):
+    """Run one 16x16x4 MFMA on row-major a (16x4), b (4x16)."""
+    _require_device(a, b)
+    c = torch.zeros((16, 16), dtype=a.dtype, device=a.device)
+    check(lib().reno_probe_mfma16(prec, _ptr(a), _ptr(b), _ptr(c), _stream()), "reno_probe_mfma16")
+    return c

@@ -1,0 +1,3 @@
+from renormalizer_amd.utils.quantity import Quantity
+from renormalizer_amd.utils.configs import (CompressCriteria, CompressConfig, OptimizeConfig,
+                                            EvolveMethod, EvolveConfig)

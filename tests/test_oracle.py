@@ -1,0 +1,218 @@
+"""Pins of the numpy oracle (CPU).
+
+Each check is chosen so that a plausible mistake in the restatement (a
+dropped term, a transposed operand, a wrong sign of dt, a wrong stopping
+index) fails it:
+
+* expm_krylov vs dense eigh exponential, N in {1,2,4,10,200,800}, real and
+  complex dt (the reference's test_krylov.py:12-39 cases) + the matvec count
+  sequence of the stopping rule;
+* hop1/hop0 vs the effective Hamiltonian assembled by brute force;
+* contract_one_site: L[i].R[i+1] = <H> on every bond (test_mp.py:51-62);
+* svd_qn: reconstruction, isometry, exact block zeros, ValueError on an
+  empty quantum-number sector;
+* TDVP-PS: exact at full bond dimension vs the dense propagator; closed forms
+  (decoupled spin: <sz>(t) = cos(2 Delta t); g = 0 Holstein: |exp(-iJt)|^2);
+  norm conservation; the reference's own integration target (3-site
+  Holstein, 2 levels, mean deviation < 1e-4 over t in [0, 5], dt = 0.4,
+  test_evolve.py:123-131) against exact diagonalisation instead of qutip.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import dense
+from oracle import tdvp as orc
+from tests import helpers
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 10, 200, 800])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_expm_krylov_dense(n, cplx):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n))
+    if cplx:
+        a = a + 1j * rng.standard_normal((n, n))
+    a = a + a.conj().T
+    v = rng.standard_normal(n) + (1j * rng.standard_normal(n) if cplx else 0)
+    for dt in (0.1, -0.05j, 0.03 - 0.02j):
+        ref = scipy.linalg.expm(dt * a) @ v
+        out, nmv = orc.expm_krylov(lambda x: a @ x, dt, v)
+        np.testing.assert_allclose(out, ref, rtol=1e-5, atol=1e-8 * max(1, np.abs(ref).max()))
+        assert nmv <= n
+        if nmv < n:  # otherwise the full-space exit (krylov.py:59-61)
+            assert nmv >= 7 and nmv % 2 == 1
+
+
+def test_expm_krylov_stop_index():
+    """Convergence is only checked at even j > 3: the count is 7, 9, 11, ..."""
+    rng = np.random.default_rng(1)
+    n = 300
+    a = np.diag(np.linspace(-1, 1, n))
+    v = rng.standard_normal(n)
+    counts = [orc.expm_krylov(lambda x: a @ x, -1j * t, v)[1] for t in (0.01, 1.0, 5.0, 20.0)]
+    assert counts[0] == 7
+    assert all(c % 2 == 1 for c in counts)
+    assert counts == sorted(counts) and counts[-1] > counts[0]
+
+
+def _dense_heff(L, W, R):
+    # H[(a,d,l),(c,e,k)] = sum_{b,f} L[a,b,c] W[b,d,e,f] R[l,f,k]
+    Ml, Dl, _ = L.shape
+    Mr, Dr, _ = R.shape
+    d = W.shape[1]
+    h = np.zeros((Ml, d, Mr, Ml, d, Mr), dtype=complex)
+    for b in range(Dl):
+        for f in range(Dr):
+            h += np.einsum("ac,de,lk->adlcek", L[:, b, :], W[b, :, :, f], R[:, f, :])
+    return h.reshape(Ml * d * Mr, -1)
+
+
+def test_hop_bruteforce():
+    rng = np.random.default_rng(2)
+    L = rng.standard_normal((3, 2, 3)) + 1j * rng.standard_normal((3, 2, 3))
+    R = rng.standard_normal((4, 3, 4)) + 1j * rng.standard_normal((4, 3, 4))
+    W = rng.standard_normal((2, 5, 5, 3))
+    C = rng.standard_normal((3, 5, 4)) + 1j * rng.standard_normal((3, 5, 4))
+    h = _dense_heff(L, W, R)
+    np.testing.assert_allclose(orc.hop1(L, W, R, C).ravel(), h @ C.ravel(), rtol=1e-12)
+    R0 = rng.standard_normal((4, 2, 4)) + 1j * rng.standard_normal((4, 2, 4))
+    S = rng.standard_normal((3, 4)) + 1j * rng.standard_normal((3, 4))
+    h0 = _dense_heff(L, np.ones((2, 1, 1, 2)) * np.eye(2)[:, None, None, :], R0)
+    np.testing.assert_allclose(orc.hop0(L, R0, S).ravel(), h0 @ S.ravel(), rtol=1e-12)
+
+
+def test_environment_energy_invariant():
+    model = helpers.sbm_model(4, 3, alpha=0.3)
+    mps, mpo = helpers.sbm_initial(model, 6, coef=1e-3)
+    st = helpers.to_state(mps)
+    w = helpers.mpo_arrays(mpo)
+    h = mpo.todense()
+    psi = _to_dense(st)
+    e_exact = np.vdot(psi, h @ psi) / np.vdot(psi, psi)
+    for bond in range(1, len(st)):
+        np.testing.assert_allclose(orc.energy_bond(st, w, bond), e_exact, rtol=1e-11)
+    np.testing.assert_allclose(orc.expectation(st, w), e_exact.real, rtol=1e-11)
+
+
+def _to_dense(st):
+    res = np.ones((1, 1))
+    for s in st.sites:
+        res = np.tensordot(res, s, axes=1).reshape(-1, s.shape[-1])
+    return res[:, 0]
+
+
+@pytest.mark.parametrize("system", ["L", "R"])
+def test_svd_qn_blocks(system):
+    rng = np.random.default_rng(3)
+    ql = rng.integers(0, 3, (12, 1))
+    qr = rng.integers(0, 3, (9, 1))
+    qntot = np.array([2])
+    mask = orc.get_qn_mask(orc.add_outer(ql, qr), qntot)
+    C = np.where(mask, rng.standard_normal((12, 9)) + 1j * rng.standard_normal((12, 9)), 0)
+    u, qnl, v, qnr = orc.svd_qn(C, ql, qr, qntot, QR=True, system=system, full_matrices=False)
+    np.testing.assert_allclose(u @ v.T, C, atol=1e-13)
+    iso = u if system == "L" else v
+    np.testing.assert_allclose(iso.conj().T @ iso, np.eye(iso.shape[1]), atol=1e-13)
+    # each new basis state lives in one sector only
+    for k, (a, b) in enumerate(zip(qnl, qnr)):
+        assert np.all(u[(ql[:, 0] != a[0]), k] == 0)
+        assert np.all(v[(qr[:, 0] != b[0]), k] == 0)
+        assert a[0] + b[0] == 2
+    us, su, _, vs, sv, _ = orc.svd_qn(C, ql, qr, qntot, full_matrices=False)
+    np.testing.assert_allclose((us * su) @ vs.T, C, atol=1e-13)
+    assert np.all(np.diff(su) <= 0)
+    with pytest.raises(ValueError):
+        orc.svd_qn(C, ql, qr, np.array([17]), QR=True, system=system, full_matrices=False)
+
+
+def _dense_sbm_evolution(model, mpo, st, dt, nsteps):
+    h = mpo.todense()
+    psi0 = _to_dense(st)
+    dims = [b.nbas for b in model.basis]
+    states = dense.propagate(h, psi0, [dt * k for k in range(nsteps + 1)])
+    sz = []
+    for psi in states:
+        r = dense.site_rdm(psi, dims, 0)
+        sz.append((r[0, 0] - r[1, 1]).real)
+    return np.array(sz), states
+
+
+def _oracle_sz(st, w, dt, nsteps):
+    out = [_sz(st)]
+    steps = []
+    for _ in range(nsteps):
+        st, ls = orc.tdvp_ps(st, w, dt)
+        out.append(_sz(st))
+        steps.append(ls)
+    return np.array(out), st, steps
+
+
+def _sz(st):
+    r = orc.calc_1site_rdm(st, 0)
+    return (r[0, 0] - r[1, 1]).real
+
+
+def test_tdvp_full_bond_dimension_exact():
+    """With M >= the exact bond dimensions the TDVP projector is the identity
+    and one step equals exp(-iH dt) up to the Krylov tolerance."""
+    model = helpers.sbm_model(3, 3, alpha=0.4)
+    mps, mpo = helpers.sbm_initial(model, 64, coef=1e-2)
+    st = helpers.to_state(mps)
+    w = helpers.mpo_arrays(mpo)
+    sz_o, st_end, _ = _oracle_sz(st, w, 0.1, 10)
+    sz_d, states = _dense_sbm_evolution(model, mpo, helpers.to_state(mps), 0.1, 10)
+    np.testing.assert_allclose(sz_o, sz_d, atol=1e-7)
+    psi = _to_dense(st_end)
+    assert abs(abs(np.vdot(psi, states[-1])) - 1) < 1e-7
+
+
+def test_tdvp_decoupled_spin_closed_form():
+    """All couplings zero, eps = 0: <sz>(t) = cos(2 Delta t)."""
+    from renormalizer_amd import Phonon, Quantity, SpinBosonModel
+    ph = [Phonon.simple_phonon(Quantity(w), Quantity(0.0), 3) for w in (0.5, 1.5)]
+    model = SpinBosonModel(Quantity(0.0), Quantity(0.7), ph)
+    mps, mpo = helpers.sbm_initial(model, 4, coef=1e-8)
+    st = helpers.to_state(mps)
+    sz, _, _ = _oracle_sz(st, helpers.mpo_arrays(mpo), 0.05, 30)
+    t = 0.05 * np.arange(31)
+    np.testing.assert_allclose(sz, np.cos(2 * 0.7 * t), atol=1e-8)
+
+
+def test_tdvp_holstein_free_exciton():
+    """g = 0: populations are |exp(-i J t)[c, i]|^2 of the hopping matrix."""
+    model = helpers.holstein_model(4, 2, j=-1.0, g=0.0)
+    mps, mpo = helpers.holstein_initial(model, 16, center=1)
+    st = helpers.to_state(mps)
+    w = helpers.mpo_arrays(mpo)
+    jm = model.j_matrix
+    dt = 0.1
+    for k in range(1, 11):
+        st, _ = orc.tdvp_ps(st, w, dt)
+        pops = helpers.e_occupations_oracle(st, model)
+        u = scipy.linalg.expm(-1j * jm * dt * k)
+        np.testing.assert_allclose(pops, np.abs(u[:, 1]) ** 2, atol=1e-8)
+        assert abs(orc.mp_norm(st) - 1) < 1e-12
+
+
+def test_tdvp_reference_holstein_target():
+    """Reference test_evolve.py:123-131 setup (3 sites, omega=1, D=1, 2 levels,
+    J=1), exact diagonalisation as the standard: mean deviation < 1e-4."""
+    from renormalizer_amd import HolsteinModel, Mol, Phonon, Quantity
+    ph = Phonon.simple_phonon(Quantity(1), Quantity(1), 2)
+    model = HolsteinModel([Mol(Quantity(0), [ph])] * 3, Quantity(1), 3)
+    mps, mpo = helpers.holstein_initial(model, 32, center=0)
+    st = helpers.to_state(mps)
+    w = helpers.mpo_arrays(mpo)
+    h = mpo.todense()
+    psi0 = _to_dense(st)
+    dims = [b.nbas for b in model.basis]
+    dt, nstep = 0.4, round(5 / 0.4)
+    exact = dense.propagate(h, psi0, [dt * k for k in range(nstep + 1)])
+    got, ref = [], []
+    for k in range(nstep + 1):
+        if k:
+            st, _ = orc.tdvp_ps(st, w, dt)
+        got.append(helpers.e_occupations_oracle(st, model))
+        ref.append([dense.site_rdm(exact[k], dims, 2 * i)[1, 1].real for i in range(3)])
+    assert np.abs(np.array(got) - np.array(ref)).mean() < 1e-4
