@@ -1,0 +1,228 @@
+#!/usr/bin/env python
+"""TDVP-PS time steps ("sweeps") per second on one MI355X.
+
+Workload (the metric's configuration): Ohmic spin-boson model, 1 spin + 64
+harmonic modes, 10 levels each, alpha = 0.3, omega_c = 10, Wang-1
+logarithmic discretisation, Delta = 1, eps = 0; MPS bond dimension M = 256;
+dt = 0.025.  Initial state: spin up x vacuum expanded along H to M = 256
+with coefficient 1e-16 (the reference's SpinBosonDynamics.init_mps), built
+on the host before timing.  One step = Mps.evolve = two half-sweeps.
+
+Both device precision modes run from the same initial MPS: "fp64"
+(complex128 everywhere) and "mixed" (complex64 environments + FP32 matrix
+cores, FP64 Krylov vectors / reductions / exponential / QR).  ``value`` is
+the mixed-mode rate; the FP64 rate, the ratio and the largest <sigma_z>
+difference between the two over all steps are reported beside it.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+# host BLAS threads for the CPU baseline: all cores we may use, set before
+# numpy is imported (the reference does the same with RENO_NUM_THREADS)
+_CORES = len(os.sched_getaffinity(0))
+os.environ.setdefault("RENO_NUM_THREADS", str(_CORES))
+for _k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
+    os.environ[_k] = os.environ["RENO_NUM_THREADS"]
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"double": 78.6, "float": 157.3}  # MI355X dense matrix FP64 / FP32
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(nmode=64, nlev=10, m=256):
+    from renormalizer_amd import (CompressConfig, CompressCriteria, EvolveConfig, EvolveMethod, Mpo, Mps,
+                                  Phonon, Quantity, SpinBosonModel)
+    from renormalizer_amd.sbm import SpectralDensityFunction
+    sdf = SpectralDensityFunction(0.3, 10.0)
+    w, c2 = sdf.Wang1(nmode)
+    omegas, dis = sdf.post_process(w, c2)
+    model = SpinBosonModel(Quantity(0.0), Quantity(1.0),
+                           [Phonon.simple_phonon(o, d, nlev) for o, d in zip(omegas, dis)])
+    mpo = Mpo(model)
+    mps = Mps.ground_state(model, False)
+    mps.compress_config = CompressConfig(CompressCriteria.fixed, max_bonddim=m)
+    mps.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps)
+    mps = mps.expand_bond_dimension(mpo, coef=1e-16, include_ex=False)
+    return model, mpo, mps
+
+
+def sigma_z(mps):
+    r = mps.calc_1site_rdm(0)[0]
+    return float((r[0, 0] - r[1, 1]).real)
+
+
+def run_mode(mps0, mpo, precision, dt, warmup, steps, profile):
+    import torch
+    from renormalizer_amd import EvolveConfig, EvolveMethod, native
+    mps = mps0.copy()
+    mps.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps, precision=precision)
+    mps.to_device()
+    history = [mps]
+    for _ in range(warmup):
+        mps = mps.evolve(mpo, dt)
+        history.append(mps)
+    torch.cuda.synchronize()
+    if profile:
+        native.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mps = mps.evolve(mpo, dt)
+        history.append(mps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = None
+    if profile:
+        native.prof_enable(False)
+        prof = native.prof_collect()
+    matvecs = [sum(m.evolve_config.local_steps) for m in history[1 + warmup:]]
+    sz = [sigma_z(m) for m in history]
+    env = mps._env_cache[2] if mps._env_cache else None
+    return {"seconds": t1 - t0, "steps_per_s": steps / (t1 - t0), "sz": sz, "prof": prof,
+            "matvecs_per_step": matvecs, "final": mps, "environ": env}
+
+
+def roofline(prof, seconds, steps):
+    """Dominant GEMM instantiation: achieved TFLOP/s from summed algorithmic
+    FLOP over summed event-timed GPU time of its launches."""
+    name, (cnt, ms, fl) = max(prof.items(), key=lambda kv: kv[1][1])
+    peak = PEAK_TFLOPS["double" if "<double" in name else "float"]
+    achieved = fl / (ms * 1e-3) / 1e12
+    total_flop = sum(v[2] for v in prof.values())
+    total_ms = sum(v[1] for v in prof.values())
+    return {
+        "bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+        "frac": round(achieved / peak, 4), "traffic": None, "kernel": name,
+        "launches": cnt, "avg_launch_us": round(ms * 1e3 / cnt, 2),
+        "gemm_flop_per_step": total_flop / steps,
+        "gemm_time_fraction": round(total_ms * 1e-3 / seconds, 3),
+        "step_tflops": round(total_flop / seconds / 1e12, 2),
+    }
+
+
+def cpu_baseline(mps_dev, env, mpo, dt, sample_reps, gemm_flop_per_step):
+    """Time the numpy oracle on full-size (256, 10, 256) site updates taken
+    from the evolved state and scale to one time step by algorithmic FLOP."""
+    import numpy as np
+
+    from oracle import tdvp as orc
+    n = len(mps_dev)
+    site = n // 2
+    sites = mps_dev.numpy_sites()
+    w = [np.asarray(x) for x in mpo]
+    L = env.read("L", site - 1).cpu().numpy().astype(np.complex128)
+    R = env.read("R", site + 1).cpu().numpy().astype(np.complex128)
+    c = sites[site]
+    Ml, d, Mr = c.shape
+    Dl, Dr = w[site].shape[0], w[site].shape[3]
+    t0 = time.perf_counter()
+    flop = 0.0
+    for _ in range(sample_reps):
+        v, k = orc.expm_krylov(lambda y: orc.hop1(L, w[site], R, y.reshape(c.shape)).ravel(),
+                               -1j * dt / 2, c.ravel())
+        v = v.reshape(c.shape)
+        qnl = np.zeros((Ml, 1), dtype=int)
+        qnr = np.zeros((d * Mr, 1), dtype=int)
+        u, _, vv, _ = orc.svd_qn(v, qnl, qnr, np.zeros(1, dtype=int), QR=True, system="R",
+                                 full_matrices=False)
+        vt = vv.T.reshape(-1, d, Mr)
+        r_new = orc.contract_one_site(R, vt, w[site], "R")
+        u_t, k0 = orc.expm_krylov(lambda y: orc.hop0(L, r_new, y.reshape(u.shape)).ravel(),
+                                  1j * dt / 2, u.ravel())
+        np.tensordot(sites[site - 1], u_t.reshape(u.shape), axes=(-1, 0))
+        f_hop1 = 8 * (Ml * Ml * Dl * d * Mr + Ml * d * Dr * Mr * Mr) + 4 * Ml * Mr * Dl * Dr * d * d
+        f_hop0 = 8 * Dl * Ml * u.shape[1] * (Ml + u.shape[1])
+        flop += k * f_hop1 + f_hop1 + k0 * f_hop0
+    t = time.perf_counter() - t0
+    rate = flop / t
+    return {
+        "value": round(rate / gemm_flop_per_step, 5), "unit": "sweeps/s",
+        "cores": int(os.environ["RENO_NUM_THREADS"]), "kind": "port",
+        "sample": (f"{sample_reps} full-size site updates (256x10x256: Lanczos expm, RQ, environment "
+                   f"update, bond expm, absorb) of the evolved M=256 state by the numpy oracle in "
+                   f"{t:.1f} s = {rate / 1e9:.1f} GFLOP/s, scaled by the per-step algorithmic GEMM FLOP"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--dt", type=float, default=0.025)
+    ap.add_argument("--bond-dim", type=int, default=256)
+    ap.add_argument("--modes", type=int, default=64)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+                    help="run one precision mode only (profiling runs)")
+    a = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the TDVP-PS sweep of a single MPS "
+                                   "is sequential along the chain (see DESIGN.md for how it would shard)",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    import torch
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    model, mpo, mps0 = build_workload(a.modes, 10, a.bond_dim)
+    log(f"initial state (host expansion) {time.perf_counter() - t0:.1f} s, bond dims max "
+        f"{max(mps0.bond_dims)}, sites {len(mps0)}")
+
+    res = {}
+    for prec in ("fp64", "mixed"):
+        if a.only and prec != a.only:
+            continue
+        log(f"running {prec}: {a.warmup} warmup + {a.steps} timed steps")
+        res[prec] = run_mode(mps0, mpo, prec, a.dt, a.warmup, a.steps, profile=True)
+        log(f"{prec}: {res[prec]['steps_per_s']:.3f} steps/s, matvecs/step "
+            f"{res[prec]['matvecs_per_step']}")
+    main_mode = "mixed" if "mixed" in res else "fp64"
+    r = res[main_mode]
+    rl = roofline(r["prof"], r["seconds"], a.steps)
+    out = {
+        "metric": "TDVP-PS sweeps/s at M=256, mixed vs FP64 GPU at <=1e-6 population error",
+        "value": round(r["steps_per_s"], 4),
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * r["seconds"] / a.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64-mfma+f64" if main_mode == "mixed" else "c128",
+        "data": "synthetic (model parameters from the workload definition; no checkpoints)",
+        "config": {"workload": f"spin-boson 1 spin + {a.modes} modes x 10 levels, Ohmic alpha=0.3 "
+                               f"omega_c=10 Wang1, M={a.bond_dim}, dt={a.dt}, TDVP-PS one-site",
+                   "mode": main_mode, "sweep": "1 step = 2 half-sweeps"},
+        "roofline": rl,
+    }
+    if "fp64" in res and "mixed" in res:
+        import numpy as np
+        err = float(np.max(np.abs(np.array(res["mixed"]["sz"]) - np.array(res["fp64"]["sz"]))))
+        out["fp64_sweeps_per_s"] = round(res["fp64"]["steps_per_s"], 4)
+        out["mixed_over_fp64"] = round(res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"], 3)
+        out["max_abs_sigma_z_err_mixed_vs_fp64"] = err
+        out["fp64_roofline"] = roofline(res["fp64"]["prof"], res["fp64"]["seconds"], a.steps)
+    if not a.no_cpu:
+        ref = res.get("fp64", r)
+        log("cpu baseline sample")
+        out["cpu_baseline"] = cpu_baseline(ref["final"], ref["environ"], mpo, a.dt, a.cpu_reps,
+                                           rl["gemm_flop_per_step"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -17,6 +17,9 @@
 //   B operand: lane l holds B[l / 16][l % 16]
 //   C/D f32  : lane l, register i holds C[4 * (l / 16) + i][l % 16]
 //   C/D f64  : lane l, register i holds C[4 * i + l / 16][l % 16]
+#include <mutex>
+#include <vector>
+
 #include "reno_common.h"
 
 namespace reno {
@@ -272,6 +275,35 @@ static void launch_r(const GemmCall& g, dim3 grid, hipStream_t s, bool ak, bool 
   }
 }
 
+// ---- optional launch timing (used by the benchmark's roofline figure) ----
+// When enabled, every GEMM launch is bracketed by two HIP events on its own
+// stream and tagged with its template instantiation and algorithmic FLOP
+// count (8 per complex MAC, 4 per real x complex MAC).
+struct ProfRec {
+  hipEvent_t a, b;
+  double flop;
+  int kind;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<ProfRec> g_prof;
+static std::vector<hipEvent_t> g_event_pool;
+
+static int kind_index(int prec, bool ak, bool bk, bool areal) {
+  return (prec == RENO_PREC_FP64 ? 0 : 8) + (ak ? 4 : 0) + (bk ? 2 : 0) + (areal ? 1 : 0);
+}
+
+static hipEvent_t pooled_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
 int launch_gemm(const GemmCall& g, hipStream_t stream) {
   if (g.M < 0 || g.N < 0 || g.K < 0 || g.batch < 0) {
     set_error("gemm: negative size M=%d N=%d K=%d batch=%d", g.M, g.N, g.K, g.batch);
@@ -305,15 +337,65 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
   dim3 grid((unsigned)tiles, (unsigned)g.batch);
   const bool ak = (g.sAk == 1);
   const bool bk = (g.sBk == 1) && !(g.sBn == 1 && g.K == 1);
-  if (g.K == 0) {
-    // C = 0 (beta = 0) or unchanged (beta = 1): run the kernel with K = 0,
-    // which stores the zero accumulators.
+  // K == 0 needs no special case: the kernel stores its zero accumulators.
+  ProfRec rec{nullptr, nullptr, 0.0, 0};
+  bool prof = false;
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (g_prof_on) {
+      rec.a = pooled_event();
+      rec.b = pooled_event();
+      prof = rec.a && rec.b;
+    }
   }
+  if (prof) (void)hipEventRecord(rec.a, stream);
   if (g.prec == RENO_PREC_FP64)
     launch_r<double>(g, grid, stream, ak, bk, areal);
   else
     launch_r<float>(g, grid, stream, ak, bk, areal);
   RENO_CHECK_LAUNCH();
+  if (prof) {
+    (void)hipEventRecord(rec.b, stream);
+    rec.flop = (areal ? 4.0 : 8.0) * (double)g.M * (double)g.N * (double)g.K * (double)g.batch;
+    rec.kind = kind_index(g.prec, ak, bk, areal);
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof.push_back(rec);
+  }
+  return RENO_OK;
+}
+
+int prof_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_on = on != 0;
+  if (on) {
+    for (auto& r : g_prof) {
+      g_event_pool.push_back(r.a);
+      g_event_pool.push_back(r.b);
+    }
+    g_prof.clear();
+  }
+  return RENO_OK;
+}
+
+// Sums, per instantiation, the launch count, GPU time (ms) and FLOP of the
+// recorded launches.  Waits for the recorded events.
+int prof_collect(int nkinds, double* ms, double* flop, int* count) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  for (int k = 0; k < nkinds; ++k) {
+    ms[k] = 0.0;
+    flop[k] = 0.0;
+    count[k] = 0;
+  }
+  for (auto& r : g_prof) {
+    RENO_HIP_CHECK(hipEventSynchronize(r.b));
+    float t = 0.f;
+    RENO_HIP_CHECK(hipEventElapsedTime(&t, r.a, r.b));
+    if (r.kind < nkinds) {
+      ms[r.kind] += t;
+      flop[r.kind] += r.flop;
+      count[r.kind] += 1;
+    }
+  }
   return RENO_OK;
 }
 
@@ -343,6 +425,12 @@ extern "C" int reno_probe_mfma16(int prec, const void* a, const void* b, void* c
                        (const float*)a, (const float*)b, (float*)c);
   RENO_CHECK_LAUNCH();
   return RENO_OK;
+}
+
+extern "C" int reno_prof_enable(int on) { return reno::prof_enable(on); }
+
+extern "C" int reno_prof_collect(int nkinds, double* ms, double* flop, int* count) {
+  return reno::prof_collect(nkinds, ms, flop, count);
 }
 
 extern "C" int reno_zgemm(int M, int N, int K, const void* A, const void* B, void* C,

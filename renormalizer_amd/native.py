@@ -32,8 +32,18 @@ EXPORTS = (
     "reno_hop_workspace_bytes", "reno_krylov_workspace_bytes",
     "reno_env_workspace_bytes", "reno_qr_workspace_bytes",
     "reno_hop_apply", "reno_env_update", "reno_env_close", "reno_expm_krylov",
-    "reno_block_qr", "reno_zgemm", "reno_probe_mfma16",
+    "reno_block_qr", "reno_zgemm", "reno_probe_mfma16", "reno_prof_enable",
+    "reno_prof_collect",
 )
+
+# demangled names of the GEMM kernel instantiations, indexed like
+# reno_prof_collect's kinds
+GEMM_KINDS = [
+    "void reno::k_cgemm<%s, %s, %s, %s>(reno::GemmCall)" % (
+        "double" if k < 8 else "float", "true" if k & 4 else "false",
+        "true" if k & 2 else "false", "true" if k & 1 else "false")
+    for k in range(16)
+]
 
 
 class RenoNativeError(RuntimeError):
@@ -96,6 +106,11 @@ def load_library(path: str = LIB_PATH):
         lib.reno_zgemm.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp]
         lib.reno_probe_mfma16.restype = c_int
         lib.reno_probe_mfma16.argtypes = [c_int, vp, vp, vp, vp]
+        lib.reno_prof_enable.restype = c_int
+        lib.reno_prof_enable.argtypes = [c_int]
+        lib.reno_prof_collect.restype = c_int
+        lib.reno_prof_collect.argtypes = [c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_double),
+                                          ctypes.POINTER(c_int)]
         _lib = lib
         return lib
 
@@ -346,6 +361,20 @@ def zgemm(A, B):
     C = torch.empty((M, N), dtype=torch.complex128, device=A.device)
     check(lib().reno_zgemm(M, N, K, _ptr(A), _ptr(B), _ptr(C), _stream()), "reno_zgemm")
     return C
+
+
+def prof_enable(on: bool):
+    check(lib().reno_prof_enable(int(bool(on))), "reno_prof_enable")
+
+
+def prof_collect():
+    """{kernel name: (launches, total ms, total algorithmic FLOP)}."""
+    n = len(GEMM_KINDS)
+    ms = (ctypes.c_double * n)()
+    fl = (ctypes.c_double * n)()
+    ct = (ctypes.c_int * n)()
+    check(lib().reno_prof_collect(n, ms, fl, ct), "reno_prof_collect")
+    return {GEMM_KINDS[k]: (ct[k], ms[k], fl[k]) for k in range(n) if ct[k]}
 
 
 def probe_mfma16(a, b, prec):
