@@ -1,21 +1,22 @@
 // Quantum-number-blocked QR / RQ of the site coefficient matrix
 // (svd_qn QR branch, reference svd_qn.py:99-240 with QR=True, economic).
 //
-// A row-major (rows x cols) block is, bit for bit, the column-major
-// (cols x rows) matrix X = block^T, so no transposes are needed:
-//   QR of block  <=>  LQ of X   (rocsolver_zgelqf + zunglq)
-//   RQ of block  <=>  QL of X   (rocsolver_zgeqlf + zungql)
-// Householder factorizations as in LAPACK: the triangular factor has a real
-// diagonal, so the result matches scipy.linalg.qr / rq up to the sign of each
-// column/row (and to rounding).  The block gather from the QN index sets and
-// the scatter into the zero-filled u / v (blockrecover, svd_qn.py:88-96) are
-// small device kernels.
+// Each block B (nrows x ncols, gathered from the QN index sets) is factored
+// by the device Householder QR of householder.hip:
+//   system "L":  B = Q R                     -> QR of X = B (column-major)
+//   system "R":  B = R Q  (scipy.linalg.rq)  -> QR of X = B^H J (columns of B^H
+//                in reverse order): X = Q' R' gives R = J R'^H J, Q = J Q'^H.
+// Householder factors with a real diagonal, so the result equals LAPACK's
+// up to the sign of each column / row (and rounding).  Blocks with more rows
+// than the register-resident panel covers go to rocSOLVER (zgelqf/zunglq,
+// zgeqlf/zungql on the same memory viewed column-major).  The scatter into the
+// zero-filled u / v (blockrecover, svd_qn.py:88-96) is one small kernel.
 #include <rocsolver/rocsolver.h>
 
 #include <map>
 #include <mutex>
 
-#include "reno_common.h"
+#include "householder.h"
 
 namespace reno {
 
@@ -48,33 +49,49 @@ __global__ void k_zero_z(double2* p, int64_t n) {
   }
 }
 
-// X[i*nc + j] = coef[rows[i]*cols_total + cols[j]]
+// X[i + j*ld] = coef[rows[ri(i,j)] * cols_total + cols[ci(i,j)]] with
+// mode 0: X (nr x nc) = B                 (ri = i, ci = j, ld = nr)
+// mode 1: X (nc x nr) = conj(B)^T, cols reversed (ri = nr-1-j, ci = i, ld = nc)
+// mode 2: X (nc x nr) = B^T row-major view (rocSOLVER path: X[j*nc + i] = B[j][i])
 __global__ void k_gather(const double2* __restrict__ coef, int64_t cols_total,
                          const int64_t* __restrict__ rows, const int64_t* __restrict__ cols,
-                         int nr, int nc, double2* __restrict__ X) {
+                         int nr, int nc, int mode, double2* __restrict__ X) {
   const int64_t total = (int64_t)nr * nc;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = t / nc, j = t - (t / nc) * nc;
-    X[t] = coef[rows[i] * cols_total + cols[j]];
+    if (mode == 0) {
+      const int64_t j = t / nr, i = t - j * nr;
+      X[t] = coef[rows[i] * cols_total + cols[j]];
+    } else if (mode == 1) {
+      const int64_t j = t / nc, i = t - j * nc;
+      const double2 b = coef[rows[nr - 1 - j] * cols_total + cols[i]];
+      X[t] = make_double2(b.x, -b.y);
+    } else {
+      const int64_t i = t / nc, j = t - i * nc;
+      X[t] = coef[rows[i] * cols_total + cols[j]];
+    }
   }
 }
 
-// dst[idx[i]*di + (off + j)*dj] = X[i*si + j*sj + x0] if cond(i,j) else 0
-// for i < ni, j < k.  cond: mode 0 -> i >= j + shift ; mode 1 -> i <= j + shift;
-// mode 2 -> always.
+// dst[idx[i]*di + (off + j)*dj] = f(X[x0 + i*si + j*sj]) for i < ni, j < k, where
+// f conjugates when cj and yields 0 unless cond(i - j): mode 0 -> i - j >= shift,
+// mode 1 -> i - j <= shift, mode 2 -> always.
 __global__ void k_scatter(const double2* __restrict__ X, int64_t x0, int64_t si, int64_t sj,
                           const int64_t* __restrict__ idx, int ni, int k, int64_t di,
-                          int64_t dj, int64_t off, int mode, int shift,
+                          int64_t dj, int64_t off, int mode, int shift, int cj,
                           double2* __restrict__ dst) {
   const int64_t total = (int64_t)ni * k;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(t / k), j = (int)(t - (t / k) * k);
     bool ok = true;
-    if (mode == 0) ok = (i >= j + shift);
-    else if (mode == 1) ok = (i <= j + shift);
-    const double2 v = ok ? X[x0 + (int64_t)i * si + (int64_t)j * sj] : make_double2(0.0, 0.0);
+    if (mode == 0) ok = (i - j >= shift);
+    else if (mode == 1) ok = (i - j <= shift);
+    double2 v = make_double2(0.0, 0.0);
+    if (ok) {
+      v = X[x0 + (int64_t)i * si + (int64_t)j * sj];
+      if (cj) v.y = -v.y;
+    }
     dst[idx[i] * di + (off + j) * dj] = v;
   }
 }
@@ -96,6 +113,57 @@ static inline int grid_for(int64_t n) {
     }                                                                          \
   } while (0)
 
+static size_t block_ws_bytes(int nr, int nc) {
+  const size_t x = align_up((size_t)nr * nc * 16);
+  const int k = nr < nc ? nr : nc;
+  const int m = nr > nc ? nr : nc;  // rows of the factored matrix (either system)
+  // X + Q (m x k) + Householder scratch, or X + tau for rocSOLVER
+  return x + align_up((size_t)m * k * 16) + qr_ws_bytes(m, k) + align_up((size_t)(k + 1) * 16);
+}
+
+// one block through rocSOLVER (rows beyond the register-resident panel)
+static int block_rocsolver(int system, hipStream_t s, const double2* coef, int cols_total,
+                           const reno_qn_block& b, int64_t kt, int64_t off, int64_t vdi, int64_t vdj,
+                           double2* u, double2* v, double2* X, double2* tau) {
+  rocblas_handle h;
+  RENO_TRY(get_handle(&h));
+  RENO_SOLVER_CHECK(rocblas_set_stream(h, s));
+  const int nr = b.nrows, nc = b.ncols;
+  const int k = nr < nc ? nr : nc;
+  rocblas_double_complex* Xr = (rocblas_double_complex*)X;
+  rocblas_double_complex* taur = (rocblas_double_complex*)tau;
+  hipLaunchKernelGGL(k_gather, dim3(grid_for((int64_t)nr * nc)), dim3(256), 0, s, coef,
+                     (int64_t)cols_total, b.rows, b.cols, nr, nc, 2, X);
+  RENO_CHECK_LAUNCH();
+  if (system == 0) {
+    // row-major block = column-major X (nc x nr): block = Q R <=> X = L Qlq
+    RENO_SOLVER_CHECK(rocsolver_zgelqf(h, nc, nr, Xr, nc, taur));
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nc * k)), dim3(256), 0, s, (const double2*)X,
+                       (int64_t)0, (int64_t)1, (int64_t)nc, b.cols, nc, k, vdi, vdj, off, 0, 0, 0, v);
+    RENO_CHECK_LAUNCH();
+    RENO_SOLVER_CHECK(rocsolver_zunglq(h, k, nr, k, Xr, nc, taur));
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nr * k)), dim3(256), 0, s, (const double2*)X,
+                       (int64_t)0, (int64_t)nc, (int64_t)1, b.rows, nr, k, kt, (int64_t)1, off, 2, 0,
+                       0, u);
+    RENO_CHECK_LAUNCH();
+  } else {
+    // block = R Q <=> X (nc x nr) = Qql L
+    RENO_SOLVER_CHECK(rocsolver_zgeqlf(h, nc, nr, Xr, nc, taur));
+    const int r0 = nc > nr ? nc - nr : 0;
+    const int q0 = nr > nc ? nr - nc : 0;
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nr * k)), dim3(256), 0, s, (const double2*)X,
+                       (int64_t)r0, (int64_t)nc, (int64_t)1, b.rows, nr, k, kt, (int64_t)1, off, 1, q0,
+                       0, u);
+    RENO_CHECK_LAUNCH();
+    RENO_SOLVER_CHECK(rocsolver_zungql(h, nc, k, k, Xr + (int64_t)q0 * nc, nc, taur));
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nc * k)), dim3(256), 0, s, (const double2*)X,
+                       (int64_t)q0 * nc, (int64_t)1, (int64_t)nc, b.cols, nc, k, vdi, vdj, off, 2, 0,
+                       0, v);
+    RENO_CHECK_LAUNCH();
+  }
+  return RENO_OK;
+}
+
 }  // namespace reno
 
 using namespace reno;
@@ -104,14 +172,12 @@ extern "C" size_t reno_qr_workspace_bytes(int rows, int cols, int nblocks,
                                           const reno_qn_block* blocks) {
   (void)rows;
   (void)cols;
-  size_t xmax = 0, kmax = 0;
+  size_t need = 0;
   for (int b = 0; b < nblocks; ++b) {
-    const size_t x = (size_t)blocks[b].nrows * blocks[b].ncols;
-    const size_t k = (size_t)(blocks[b].nrows < blocks[b].ncols ? blocks[b].nrows : blocks[b].ncols);
-    if (x > xmax) xmax = x;
-    if (k > kmax) kmax = k;
+    const size_t x = block_ws_bytes(blocks[b].nrows, blocks[b].ncols);
+    if (x > need) need = x;
   }
-  return align_up(xmax * 16) + align_up((kmax + 1) * 16);
+  return need;
 }
 
 extern "C" int reno_block_qr(int system, int rows, int cols, const void* coef, int nblocks,
@@ -144,68 +210,57 @@ extern "C" int reno_block_qr(int system, int rows, int cols, const void* coef, i
     return RENO_ENOSPACE;
   }
   hipStream_t s = (hipStream_t)stream;
-  rocblas_handle h;
-  RENO_TRY(get_handle(&h));
-  RENO_SOLVER_CHECK(rocblas_set_stream(h, s));
-
-  size_t xmax = 0;
-  for (int b = 0; b < nblocks; ++b) {
-    const size_t x = (size_t)blocks[b].nrows * blocks[b].ncols;
-    if (x > xmax) xmax = x;
-  }
-  double2* X = (double2*)ws;
-  double2* tau = (double2*)((char*)ws + align_up(xmax * 16));
-  rocblas_double_complex* Xr = (rocblas_double_complex*)X;
-  rocblas_double_complex* taur = (rocblas_double_complex*)tau;
-
-  hipLaunchKernelGGL(k_zero_z, dim3(grid_for((int64_t)rows * kt)), dim3(256), 0, s,
-                     (double2*)u, (int64_t)rows * kt);
-  hipLaunchKernelGGL(k_zero_z, dim3(grid_for((int64_t)cols * kt)), dim3(256), 0, s,
-                     (double2*)v, (int64_t)cols * kt);
+  const double2* cf = (const double2*)coef;
+  double2* du = (double2*)u;
+  double2* dv = (double2*)v;
+  hipLaunchKernelGGL(k_zero_z, dim3(grid_for((int64_t)rows * kt)), dim3(256), 0, s, du,
+                     (int64_t)rows * kt);
+  hipLaunchKernelGGL(k_zero_z, dim3(grid_for((int64_t)cols * kt)), dim3(256), 0, s, dv,
+                     (int64_t)cols * kt);
   RENO_CHECK_LAUNCH();
-
   // u is (rows x kt); v is (cols x kt), or its transpose (kt x cols)
   const int64_t vdi = v_transposed ? 1 : kt;
   const int64_t vdj = v_transposed ? (int64_t)cols : 1;
   int64_t off = 0;
-  for (int b = 0; b < nblocks; ++b) {
-    const int nr = blocks[b].nrows, nc = blocks[b].ncols;
+  for (int ib = 0; ib < nblocks; ++ib) {
+    const reno_qn_block& b = blocks[ib];
+    const int nr = b.nrows, nc = b.ncols;
     const int k = nr < nc ? nr : nc;
-    hipLaunchKernelGGL(k_gather, dim3(grid_for((int64_t)nr * nc)), dim3(256), 0, s,
-                       (const double2*)coef, (int64_t)cols, blocks[b].rows, blocks[b].cols, nr,
-                       nc, X);
-    RENO_CHECK_LAUNCH();
-    if (system == 0) {
-      // block = Q R  <=>  X (nc x nr, col-major) = L Qlq,  Q = Qlq^T, R = L^T
-      RENO_SOLVER_CHECK(rocsolver_zgelqf(h, nc, nr, Xr, nc, taur));
-      // v = R^T = L: v[cols[i], off+j] = X[i + j*nc] for i >= j
-      hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nc * k)), dim3(256), 0, s,
-                         (const double2*)X, (int64_t)0, (int64_t)1, (int64_t)nc, blocks[b].cols,
-                         nc, k, vdi, vdj, off, 0, 0, (double2*)v);
-      RENO_CHECK_LAUNCH();
-      RENO_SOLVER_CHECK(rocsolver_zunglq(h, k, nr, k, Xr, nc, taur));
-      // u = Q = Qlq^T: u[rows[i], off+j] = X[j + i*nc]
-      hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nr * k)), dim3(256), 0, s,
-                         (const double2*)X, (int64_t)0, (int64_t)nc, (int64_t)1, blocks[b].rows,
-                         nr, k, kt, (int64_t)1, off, 2, 0, (double2*)u);
-      RENO_CHECK_LAUNCH();
-    } else {
-      // block = R Q  <=>  X (nc x nr, col-major) = Qql L,  R = L^T, Q = Qql^T
-      RENO_SOLVER_CHECK(rocsolver_zgeqlf(h, nc, nr, Xr, nc, taur));
-      const int r0 = nc > nr ? nc - nr : 0;
-      const int q0 = nr > nc ? nr - nc : 0;
-      // u = R = L^T: u[rows[i], off+j] = X[(r0 + j) + i*nc] for i <= j + q0
-      hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nr * k)), dim3(256), 0, s,
-                         (const double2*)X, (int64_t)r0, (int64_t)nc, (int64_t)1, blocks[b].rows,
-                         nr, k, kt, (int64_t)1, off, 1, q0, (double2*)u);
-      RENO_CHECK_LAUNCH();
-      RENO_SOLVER_CHECK(rocsolver_zungql(h, nc, k, k, Xr + (int64_t)q0 * nc, nc, taur));
-      // v = Q^T^T = Qql: v[cols[i], off+j] = X[i + (q0 + j)*nc]
-      hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nc * k)), dim3(256), 0, s,
-                         (const double2*)X, (int64_t)q0 * nc, (int64_t)1, (int64_t)nc,
-                         blocks[b].cols, nc, k, vdi, vdj, off, 2, 0, (double2*)v);
-      RENO_CHECK_LAUNCH();
+    char* p = (char*)ws;
+    double2* X = (double2*)p;
+    p += align_up((size_t)nr * nc * 16);
+    const int m = system == 0 ? nr : nc;  // rows of the factored matrix
+    const int n = system == 0 ? nc : nr;
+    double2* Q = (double2*)p;
+    p += align_up((size_t)m * k * 16);
+    if (!qr_supported(m, n)) {
+      RENO_TRY(block_rocsolver(system, s, cf, cols, b, kt, off, vdi, vdj, du, dv, X, Q));
+      off += k;
+      continue;
     }
+    hipLaunchKernelGGL(k_gather, dim3(grid_for((int64_t)nr * nc)), dim3(256), 0, s, cf,
+                       (int64_t)cols, b.rows, b.cols, nr, nc, system == 0 ? 0 : 1, X);
+    RENO_CHECK_LAUNCH();
+    RENO_TRY(qr_householder(s, m, n, X, Q, p));
+    if (system == 0) {
+      // u[rows[i], off+j] = Q[i + j*m];  v[cols[c], off+j] = R[j, c] = X[j + c*m], j <= c
+      hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nr * k)), dim3(256), 0, s,
+                         (const double2*)Q, (int64_t)0, (int64_t)1, (int64_t)m, b.rows, nr, k, kt,
+                         (int64_t)1, off, 2, 0, 0, du);
+      hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nc * k)), dim3(256), 0, s,
+                         (const double2*)X, (int64_t)0, (int64_t)m, (int64_t)1, b.cols, nc, k, vdi,
+                         vdj, off, 0, 0, 0, dv);
+    } else {
+      // u[rows[r], off+a] = conj(R'[k-1-a, nr-1-r]), valid for r - a <= nr - k
+      hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nr * k)), dim3(256), 0, s,
+                         (const double2*)X, (int64_t)(k - 1) + (int64_t)(nr - 1) * m, (int64_t)-m,
+                         (int64_t)-1, b.rows, nr, k, kt, (int64_t)1, off, 1, nr - k, 1, du);
+      // v[cols[c], off+a] = conj(Q'[c, k-1-a])
+      hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nc * k)), dim3(256), 0, s,
+                         (const double2*)Q, (int64_t)(k - 1) * m, (int64_t)1, (int64_t)-m, b.cols,
+                         nc, k, vdi, vdj, off, 2, 0, 1, dv);
+    }
+    RENO_CHECK_LAUNCH();
     off += k;
   }
   return RENO_OK;

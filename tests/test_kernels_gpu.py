@@ -179,7 +179,9 @@ def test_expm_krylov_bond(dev):
 
 
 @pytest.mark.parametrize("system", ["L", "R"])
-@pytest.mark.parametrize("rows,cols,nblk", [(20, 8, 1), (8, 20, 1), (64, 32, 2), (30, 30, 2), (6, 3, 1), (3, 6, 1)])
+@pytest.mark.parametrize("rows,cols,nblk", [(20, 8, 1), (8, 20, 1), (64, 32, 2), (30, 30, 2), (6, 3, 1), (3, 6, 1),
+                                            (2560, 256, 1), (256, 2560, 1), (640, 64, 2), (1, 7, 1), (7, 1, 1),
+                                            (3100, 40, 1)])
 def test_block_qr(dev, system, rows, cols, nblk):
     print(f"[block_qr {system} {rows}x{cols} blocks={nblk}]", flush=True)
     rng = np.random.default_rng(rows * 31 + cols + nblk)
@@ -206,9 +208,34 @@ def test_block_qr(dev, system, rows, cols, nblk):
     iso = u if system == "L" else v
     np.testing.assert_allclose(iso.conj().T @ iso, np.eye(iso.shape[1]), atol=1e-13)
     # equal to scipy's factors up to a sign per column (Householder, real diagonal)
-    for k in range(u.shape[1]):
+    for k in range(min(u.shape[1], 64)):
         j = np.argmax(np.abs(uo[:, k]))
         ph = u[j, k] / uo[j, k]
         assert abs(abs(ph) - 1) < 1e-10
         np.testing.assert_allclose(u[:, k], ph * uo[:, k], atol=1e-11)
         np.testing.assert_allclose(v[:, k], vo[:, k] / ph, atol=1e-11)
+
+
+@pytest.mark.parametrize("system", ["L", "R"])
+def test_block_qr_rank_deficient(dev, system):
+    """Exactly zero and repeated columns: still an isometry and an exact
+    reconstruction (Householder reflectors with tau = 0)."""
+    print(f"[block_qr rank deficient {system}]", flush=True)
+    rng = np.random.default_rng(9)
+    C = _rand(rng, (300, 40))
+    C[:, 5] = 0
+    C[:, 7] = C[:, 3]
+    C[:, 20:24] = 1e-17 * C[:, 30:34]
+    if system == "R":
+        C = C.T.copy()
+    rows, cols = C.shape
+    blocks = [(np.arange(rows), np.arange(cols))]
+    u, v = native.block_qr(system, _t(C, dev), blocks)
+    u = u.cpu().numpy()
+    v = v.cpu().numpy()
+    assert np.linalg.norm(u @ v.T - C) / np.linalg.norm(C) < 1e-14
+    iso = u if system == "L" else v
+    np.testing.assert_allclose(iso.conj().T @ iso, np.eye(iso.shape[1]), atol=1e-13)
+    # v^T layout option gives the transpose of the same factor
+    u2, vt = native.block_qr(system, _t(C, dev), blocks, v_transposed=True)
+    np.testing.assert_array_equal(vt.cpu().numpy(), v.T)
