@@ -11,8 +11,10 @@ on the host before timing.  One step = Mps.evolve = two half-sweeps.
 Both device precision modes run from the same initial MPS: "fp64"
 (complex128 everywhere) and "mixed" (complex64 environments + FP32 matrix
 cores, FP64 Krylov vectors / reductions / exponential / QR).  ``value`` is
-the mixed-mode rate; the FP64 rate, the ratio and the largest <sigma_z>
-difference between the two over all steps are reported beside it.
+the rate of the faster mode whose largest <sigma_z> difference from the FP64
+trajectory stays within 1e-6 (the FP64 mode itself when the mixed mode
+misses that bound); both rates, their ratio and the difference are reported
+beside it.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -33,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"double": 78.6, "float": 157.3}  # MI355X dense matrix FP64 / FP32
+ACCURACY_BOUND = 1e-6  # largest allowed |<sigma_z>_mixed - <sigma_z>_fp64| over the run
 
 
 def log(msg):
@@ -168,8 +171,8 @@ def main():
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the TDVP-PS sweep of a single MPS "
-                                   "is sequential along the chain (see DESIGN.md for how it would shard)",
+        print(json.dumps({"error": "this build runs on one GPU only: the TDVP-PS sweep of a single MPS "
+                                   "is sequential along the chain and is measured on one device",
                           "n_gpus": a.gpus}))
         sys.exit(2)
 
@@ -188,7 +191,17 @@ def main():
         res[prec] = run_mode(mps0, mpo, prec, a.dt, a.warmup, a.steps, profile=True)
         log(f"{prec}: {res[prec]['steps_per_s']:.3f} steps/s, matvecs/step "
             f"{res[prec]['matvecs_per_step']}")
-    main_mode = "mixed" if "mixed" in res else "fp64"
+    err = None
+    if "fp64" in res and "mixed" in res:
+        import numpy as np
+        err = float(np.max(np.abs(np.array(res["mixed"]["sz"]) - np.array(res["fp64"]["sz"]))))
+    if "fp64" not in res:
+        main_mode = "mixed"
+    elif "mixed" in res and err is not None and err <= ACCURACY_BOUND \
+            and res["mixed"]["steps_per_s"] > res["fp64"]["steps_per_s"]:
+        main_mode = "mixed"
+    else:
+        main_mode = "fp64"
     r = res[main_mode]
     rl = roofline(r["prof"], r["seconds"], a.steps)
     out = {
@@ -209,13 +222,14 @@ def main():
                    "mode": main_mode, "sweep": "1 step = 2 half-sweeps"},
         "roofline": rl,
     }
-    if "fp64" in res and "mixed" in res:
-        import numpy as np
-        err = float(np.max(np.abs(np.array(res["mixed"]["sz"]) - np.array(res["fp64"]["sz"]))))
+    if err is not None:
         out["fp64_sweeps_per_s"] = round(res["fp64"]["steps_per_s"], 4)
+        out["mixed_sweeps_per_s"] = round(res["mixed"]["steps_per_s"], 4)
         out["mixed_over_fp64"] = round(res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"], 3)
         out["max_abs_sigma_z_err_mixed_vs_fp64"] = err
-        out["fp64_roofline"] = roofline(res["fp64"]["prof"], res["fp64"]["seconds"], a.steps)
+        out["mixed_within_bound"] = err <= ACCURACY_BOUND
+        other = "mixed" if main_mode == "fp64" else "fp64"
+        out[other + "_roofline"] = roofline(res[other]["prof"], res[other]["seconds"], a.steps)
     if not a.no_cpu:
         ref = res.get("fp64", r)
         log("cpu baseline sample")

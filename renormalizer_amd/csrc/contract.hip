@@ -5,6 +5,8 @@
 // lib.py:200-248).  Every stage is one launch of the GEMM in gemm.hip; the MPO
 // stage is a batched GEMM with a real A operand (the site MPO, permuted once
 // into "Wl" layout for the left-type contractions).
+#include <algorithm>
+
 #include "contract.h"
 
 namespace reno {
@@ -79,22 +81,29 @@ HopLayout hop_layout(const reno_hop_desc* op) {
   HopLayout h{};
   const size_t ce = elem_bytes(op->prec);
   const int64_t Ml = op->Ml, d = op->d, Mr = op->Mr, Dl = op->Dl, Dr = op->Dr;
+  const int p = op->prec;
   if (op->nsite == 1) {
     h.wl_bytes = align_up((size_t)(Dl * d * d * Dr) * real_bytes(op->prec));
     h.t1_bytes = align_up((size_t)(Ml * Dl * d * Mr) * ce);
     h.t2_bytes = align_up((size_t)(Ml * d * Dr * Mr) * ce);
+    h.part_bytes = std::max({splitk_bytes(p, Ml * Dl, d * Mr, Ml, 1),
+                             splitk_bytes(p, d * Dr, Mr, Dl * d, Ml),
+                             splitk_bytes(p, Ml * d, Mr, Dr * Mr, 1)});
   } else {
     h.wl_bytes = 0;
     h.t1_bytes = align_up((size_t)(Ml * Dl * Mr) * ce);
     h.t2_bytes = 0;
+    h.part_bytes = std::max(splitk_bytes(p, Ml * Dl, Mr, Ml, 1), splitk_bytes(p, Ml, Mr, Dl * Mr, 1));
   }
-  h.total = h.wl_bytes + h.t1_bytes + h.t2_bytes;
+  h.part_bytes = align_up(h.part_bytes);
+  h.total = h.wl_bytes + h.t1_bytes + h.t2_bytes + h.part_bytes;
   return h;
 }
 
 // out = H_eff * cin with pre-permuted Wl (nsite 1) and scratch t1/t2.
 int hop_apply_core(const reno_hop_desc* op, const void* Wl, const void* cin, void* cout,
-                   void* t1, void* t2, hipStream_t s) {
+                   void* t1, void* t2, hipStream_t s, const int* skip, void* part,
+                   size_t part_bytes) {
   const int prec = op->prec;
   const int ck = cplx_kind(prec);
   const int64_t Ml = op->Ml, d = op->d, Mr = op->Mr, Dl = op->Dl, Dr = op->Dr;
@@ -105,6 +114,9 @@ int hop_apply_core(const reno_hop_desc* op, const void* Wl, const void* cin, voi
     g1.A = op->L; g1.a_kind = ck; g1.sAm = Ml; g1.sAk = 1;
     g1.B = cin; g1.b_kind = ST_Z128; g1.sBk = d * Mr; g1.sBn = 1;
     g1.C = t1; g1.c_kind = ck; g1.sCm = d * Mr; g1.sCn = 1;
+    g1.skip = skip;
+    g1.part = part;
+    g1.part_bytes = part_bytes;
     RENO_TRY(launch_gemm(g1, s));
     // S2: T2[a,(dd,f),k] = Wl[(dd,f),(b,e)] T1[a,(b,e),k]   (batched over a)
     GemmCall g2{};
@@ -112,6 +124,9 @@ int hop_apply_core(const reno_hop_desc* op, const void* Wl, const void* cin, voi
     g2.A = Wl; g2.a_kind = real_kind(prec); g2.sAm = Dl * d; g2.sAk = 1; g2.sAb = 0;
     g2.B = t1; g2.b_kind = ck; g2.sBk = Mr; g2.sBn = 1; g2.sBb = Dl * d * Mr;
     g2.C = t2; g2.c_kind = ck; g2.sCm = Mr; g2.sCn = 1; g2.sCb = d * Dr * Mr;
+    g2.skip = skip;
+    g2.part = part;
+    g2.part_bytes = part_bytes;
     RENO_TRY(launch_gemm(g2, s));
     // S3: out[(a,dd),l] = T2[(a,dd),(f,k)] R[l,(f,k)]
     GemmCall g3{};
@@ -119,6 +134,9 @@ int hop_apply_core(const reno_hop_desc* op, const void* Wl, const void* cin, voi
     g3.A = t2; g3.a_kind = ck; g3.sAm = Dr * Mr; g3.sAk = 1;
     g3.B = op->R; g3.b_kind = ck; g3.sBk = 1; g3.sBn = Dr * Mr;
     g3.C = cout; g3.c_kind = ST_Z128; g3.sCm = Mr; g3.sCn = 1;
+    g3.skip = skip;
+    g3.part = part;
+    g3.part_bytes = part_bytes;
     RENO_TRY(launch_gemm(g3, s));
   } else {
     const int64_t D = Dl;
@@ -128,6 +146,9 @@ int hop_apply_core(const reno_hop_desc* op, const void* Wl, const void* cin, voi
     g1.A = op->L; g1.a_kind = ck; g1.sAm = Ml; g1.sAk = 1;
     g1.B = cin; g1.b_kind = ST_Z128; g1.sBk = Mr; g1.sBn = 1;
     g1.C = t1; g1.c_kind = ck; g1.sCm = Mr; g1.sCn = 1;
+    g1.skip = skip;
+    g1.part = part;
+    g1.part_bytes = part_bytes;
     RENO_TRY(launch_gemm(g1, s));
     // S2: out[a,l] = T[a,(b,k)] R[l,(b,k)]
     GemmCall g2{};
@@ -135,16 +156,25 @@ int hop_apply_core(const reno_hop_desc* op, const void* Wl, const void* cin, voi
     g2.A = t1; g2.a_kind = ck; g2.sAm = D * Mr; g2.sAk = 1;
     g2.B = op->R; g2.b_kind = ck; g2.sBk = 1; g2.sBn = D * Mr;
     g2.C = cout; g2.c_kind = ST_Z128; g2.sCm = Mr; g2.sCn = 1;
+    g2.skip = skip;
+    g2.part = part;
+    g2.part_bytes = part_bytes;
     RENO_TRY(launch_gemm(g2, s));
   }
   return RENO_OK;
+}
+
+static size_t env_part_bytes(int p, int64_t Ml, int64_t d, int64_t Mr, int64_t Dl, int64_t Dr) {
+  return align_up(std::max({splitk_bytes(p, Ml * Dl, d * Mr, Ml, 1), splitk_bytes(p, d * Dr, Mr, Dl * d, Ml),
+                            splitk_bytes(p, Mr, Dr * Mr, Ml * d, 1), splitk_bytes(p, Ml * d, Mr, Mr, Dr),
+                            splitk_bytes(p, Dl * d, Mr, d * Dr, Ml), splitk_bytes(p, Ml, Ml, d * Mr, Dl)}));
 }
 
 size_t env_ws_bytes(int prec, int64_t Ml, int64_t d, int64_t Mr, int64_t Dl, int64_t Dr) {
   const size_t ce = elem_bytes(prec);
   size_t wl = align_up((size_t)(Dl * d * d * Dr) * real_bytes(prec));
   size_t y = align_up((size_t)(Ml * d * Mr * (Dl > Dr ? Dl : Dr)) * ce);
-  return wl + 2 * y;
+  return wl + 2 * y + env_part_bytes(prec, Ml, d, Mr, Dl, Dr);
 }
 
 int env_update_core(int domain, int prec, int64_t Ml, int64_t d, int64_t Mr, int64_t Dl,
@@ -158,6 +188,9 @@ int env_update_core(int domain, int prec, int64_t Ml, int64_t d, int64_t Mr, int
   void* y = p;
   p += align_up((size_t)(Ml * d * Mr * (Dl > Dr ? Dl : Dr)) * ce);
   void* z = p;
+  p += align_up((size_t)(Ml * d * Mr * (Dl > Dr ? Dl : Dr)) * ce);
+  void* part = p;
+  const size_t part_bytes = env_part_bytes(prec, Ml, d, Mr, Dl, Dr);
   // bra operand: conj(ms) unless an explicit (already conjugated) bra is given
   const void* bra = ms_conj ? ms_conj : ms;
   const int conj_bra = ms_conj ? 0 : 1;
@@ -169,6 +202,8 @@ int env_update_core(int domain, int prec, int64_t Ml, int64_t d, int64_t Mr, int
     g1.A = env_in; g1.a_kind = ck; g1.sAm = Ml; g1.sAk = 1;
     g1.B = ms; g1.b_kind = ST_Z128; g1.sBk = d * Mr; g1.sBn = 1;
     g1.C = y; g1.c_kind = ck; g1.sCm = d * Mr; g1.sCn = 1;
+    g1.part = part;
+    g1.part_bytes = part_bytes;
     RENO_TRY(launch_gemm(g1, s));
     // S2: Z[a,(dd,g),h] = Wl[(dd,g),(b,e)] Y[a,(b,e),h]
     GemmCall g2{};
@@ -176,6 +211,8 @@ int env_update_core(int domain, int prec, int64_t Ml, int64_t d, int64_t Mr, int
     g2.A = wl; g2.a_kind = real_kind(prec); g2.sAm = Dl * d; g2.sAk = 1; g2.sAb = 0;
     g2.B = y; g2.b_kind = ck; g2.sBk = Mr; g2.sBn = 1; g2.sBb = Dl * d * Mr;
     g2.C = z; g2.c_kind = ck; g2.sCm = Mr; g2.sCn = 1; g2.sCb = d * Dr * Mr;
+    g2.part = part;
+    g2.part_bytes = part_bytes;
     RENO_TRY(launch_gemm(g2, s));
     // S3: out[f,(g,h)] = bra*[(a,dd),f] Z[(a,dd),(g,h)]
     GemmCall g3{};
@@ -183,6 +220,8 @@ int env_update_core(int domain, int prec, int64_t Ml, int64_t d, int64_t Mr, int
     g3.A = bra; g3.a_kind = ST_Z128; g3.sAm = 1; g3.sAk = Mr; g3.conjA = conj_bra;
     g3.B = z; g3.b_kind = ck; g3.sBk = Dr * Mr; g3.sBn = 1;
     g3.C = env_out; g3.c_kind = ck; g3.sCm = Dr * Mr; g3.sCn = 1;
+    g3.part = part;
+    g3.part_bytes = part_bytes;
     RENO_TRY(launch_gemm(g3, s));
   } else {
     // S1: Y[(h,e), b, a] = ms[(h,e),c] R[a,b,c]   (batched over b)
@@ -191,6 +230,8 @@ int env_update_core(int domain, int prec, int64_t Ml, int64_t d, int64_t Mr, int
     g1.A = ms; g1.a_kind = ST_Z128; g1.sAm = Mr; g1.sAk = 1; g1.sAb = 0;
     g1.B = env_in; g1.b_kind = ck; g1.sBk = 1; g1.sBn = Dr * Mr; g1.sBb = Mr;
     g1.C = y; g1.c_kind = ck; g1.sCm = Dr * Mr; g1.sCn = 1; g1.sCb = Mr;
+    g1.part = part;
+    g1.part_bytes = part_bytes;
     RENO_TRY(launch_gemm(g1, s));
     // S2: Z[h,(g,dd),a] = W[(g,dd),(e,b)] Y[h,(e,b),a]   (batched over h;
     //     W viewed as the (Dl*d) x (d*Dr) matrix it already is)
@@ -199,6 +240,8 @@ int env_update_core(int domain, int prec, int64_t Ml, int64_t d, int64_t Mr, int
     g2.A = W; g2.a_kind = real_kind(prec); g2.sAm = d * Dr; g2.sAk = 1; g2.sAb = 0;
     g2.B = y; g2.b_kind = ck; g2.sBk = Mr; g2.sBn = 1; g2.sBb = d * Dr * Mr;
     g2.C = z; g2.c_kind = ck; g2.sCm = Mr; g2.sCn = 1; g2.sCb = Dl * d * Mr;
+    g2.part = part;
+    g2.part_bytes = part_bytes;
     RENO_TRY(launch_gemm(g2, s));
     // S3: out[f,g,h] = bra*[f,(dd,a)] Z[h,g,(dd,a)]   (batched over g)
     GemmCall g3{};
@@ -206,6 +249,8 @@ int env_update_core(int domain, int prec, int64_t Ml, int64_t d, int64_t Mr, int
     g3.A = bra; g3.a_kind = ST_Z128; g3.sAm = d * Mr; g3.sAk = 1; g3.sAb = 0; g3.conjA = conj_bra;
     g3.B = z; g3.b_kind = ck; g3.sBk = 1; g3.sBn = Dl * d * Mr; g3.sBb = d * Mr;
     g3.C = env_out; g3.c_kind = ck; g3.sCm = Dl * Ml; g3.sCn = 1; g3.sCb = Ml;
+    g3.part = part;
+    g3.part_bytes = part_bytes;
     RENO_TRY(launch_gemm(g3, s));
   }
   return RENO_OK;
@@ -285,8 +330,9 @@ extern "C" int reno_hop_apply(const reno_hop_desc* op, const void* c_in, void* c
   void* wl = p;
   void* t1 = p + h.wl_bytes;
   void* t2 = p + h.wl_bytes + h.t1_bytes;
+  void* part = p + h.wl_bytes + h.t1_bytes + h.t2_bytes;
   if (op->nsite == 1) RENO_TRY(permute_w(op->prec, op->W, wl, op->Dl, op->d, op->Dr, s));
-  return hop_apply_core(op, wl, c_in, c_out, t1, t2, s);
+  return hop_apply_core(op, wl, c_in, c_out, t1, t2, s, nullptr, part, h.part_bytes);
 }
 
 extern "C" size_t reno_env_workspace_bytes(int prec, int Ml, int d, int Mr, int Dl, int Dr) {

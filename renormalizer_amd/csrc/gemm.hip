@@ -18,6 +18,7 @@
 //   C/D f32  : lane l, register i holds C[4 * (l / 16) + i][l % 16]
 //   C/D f64  : lane l, register i holds C[4 * i + l / 16][l % 16]
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "reno_common.h"
@@ -114,6 +115,9 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
   __shared__ R sA[2][kBK][kBM + kPad];
   __shared__ R sB[2][kBK][kBN + kPad];
   typedef typename Mfma<R>::acc acc_t;
+  if (g.skip && *g.skip) return;
+  if (g.exec && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0)
+    *g.exec = 1;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -121,15 +125,24 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
   const int wm = wave >> 1;
   const int wn = wave & 1;
 
+  // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
+  // give each XCD a contiguous run of tiles (shared A row panels in its L2)
   const int tilesN = (g.N + kBN - 1) / kBN;
-  const int tm = blockIdx.x / tilesN;
-  const int tn = blockIdx.x - tm * tilesN;
+  const int ntiles = gridDim.x;
+  const int t8 = (ntiles / 8) * 8;
+  const int bid = blockIdx.x;
+  const int tile = bid < t8 ? (bid % 8) * (ntiles / 8) + bid / 8 : bid;
+  const int tm = tile / tilesN;
+  const int tn = tile - tm * tilesN;
   const int m0 = tm * kBM;
   const int n0 = tn * kBN;
   const int64_t bat = blockIdx.y;
   const int64_t aoff = bat * g.sAb;
   const int64_t boff = bat * g.sBb;
-  const int64_t coff = bat * g.sCb;
+  // split-K: blockIdx.z owns K range [kb, ke) and writes its own partial
+  const int kb = blockIdx.z * g.kchunk;
+  const int ke = min(g.K, kb + g.kchunk);
+  const int64_t coff = bat * g.sCb + (int64_t)blockIdx.z * gridDim.y * g.sCb;
 
   acc_t cr[2][2], ci[2][2];
 #pragma unroll
@@ -172,7 +185,7 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
       a_coord(e, mm, kk);
       const int gm = m0 + mm, gk = k0 + kk;
       R re = (R)0, im = (R)0;
-      if (gm < g.M && gk < g.K)
+      if (gm < g.M && gk < ke)
         load_elem<R>(g.A, g.a_kind, aoff + (int64_t)gm * g.sAm + (int64_t)gk * g.sAk, re, im);
       pa_re[i] = re;
       pa_im[i] = g.conjA ? -im : im;
@@ -182,7 +195,7 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
       const int gk2 = k0 + kk;
       re = (R)0;
       im = (R)0;
-      if (gn < g.N && gk2 < g.K)
+      if (gn < g.N && gk2 < ke)
         load_elem<R>(g.B, g.b_kind, boff + (int64_t)gk2 * g.sBk + (int64_t)gn * g.sBn, re, im);
       pb_re[i] = re;
       pb_im[i] = g.conjB ? -im : im;
@@ -207,12 +220,12 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
   const int r16 = lane & 15;
   const int kq = lane >> 4;
 
-  gload(0);
-  for (int k0 = 0; k0 < g.K; k0 += kBK) {
+  gload(kb);
+  for (int k0 = kb; k0 < ke; k0 += kBK) {
     __syncthreads();
     sstore();
     __syncthreads();
-    if (k0 + kBK < g.K) gload(k0 + kBK);
+    if (k0 + kBK < ke) gload(k0 + kBK);
 #pragma unroll
     for (int ks = 0; ks < kBK / 4; ++ks) {
       const int kr = ks * 4 + kq;
@@ -275,19 +288,65 @@ static void launch_r(const GemmCall& g, dim3 grid, hipStream_t s, bool ak, bool 
   }
 }
 
+// ---- split-K -------------------------------------------------------------
+// Split K into chunks (multiples of the K tile) when the output tile grid
+// would leave most of the 256 CUs idle; each chunk writes its own partial and
+// k_split_reduce sums them in a fixed order (deterministic, no atomics).
+static void split_plan(int M, int N, int K, int batch, int* nsplit, int* kchunk) {
+  const int64_t tiles = (int64_t)((M + kBM - 1) / kBM) * ((N + kBN - 1) / kBN) * batch;
+  *nsplit = 1;
+  *kchunk = K > 0 ? K : 1;
+  const int64_t target = 512;  // about two workgroups per CU
+  if (tiles >= target / 2 || K < 256) return;
+  int s = (int)((target + tiles - 1) / tiles);
+  const int max_s = K / 128;  // keep >= 128 of K per chunk
+  if (s > max_s) s = max_s;
+  if (s < 2) return;
+  int chunk = (K + s - 1) / s;
+  chunk = (chunk + kBK - 1) / kBK * kBK;
+  *kchunk = chunk;
+  *nsplit = (K + chunk - 1) / chunk;
+}
+
+// C[b](m,n) (+)= sum_s P[s][b][m][n]
+template <typename R>
+__global__ void k_split_reduce(const typename std::conditional<sizeof(R) == 8, double2, float2>::type* __restrict__ P,
+                               int nsplit, int batch, int M, int N, void* C, int c_kind,
+                               int64_t sCm, int64_t sCb, int beta, const int* skip) {
+  if (skip && *skip) return;
+  const int64_t per = (int64_t)M * N;
+  const int64_t total = per * batch;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    R re = (R)0, im = (R)0;
+    for (int s = 0; s < nsplit; ++s) {
+      const auto v = P[(int64_t)s * total + t];
+      re += v.x;
+      im += v.y;
+    }
+    const int64_t b = t / per, r = t - b * per;
+    const int64_t m = r / N, n = r - m * N;
+    store_elem<R>(C, c_kind, b * sCb + m * sCm + n, re, im, beta);
+  }
+}
+
 // ---- optional launch timing (used by the benchmark's roofline figure) ----
 // When enabled, every GEMM launch is bracketed by two HIP events on its own
 // stream and tagged with its template instantiation and algorithmic FLOP
-// count (8 per complex MAC, 4 per real x complex MAC).
+// count (8 per complex MAC, 4 per real x complex MAC); a per-launch marker
+// in device memory tells launches that ran from ones cancelled on the device.
 struct ProfRec {
   hipEvent_t a, b;
   double flop;
   int kind;
+  int slot;
 };
 static std::mutex g_prof_mu;
 static bool g_prof_on = false;
 static std::vector<ProfRec> g_prof;
 static std::vector<hipEvent_t> g_event_pool;
+static constexpr size_t kExecSlots = 1 << 20;
+static int* g_exec = nullptr;  // kExecSlots markers, allocated on first enable
 
 static int kind_index(int prec, bool ak, bool bk, bool areal) {
   return (prec == RENO_PREC_FP64 ? 0 : 8) + (ak ? 4 : 0) + (bk ? 2 : 0) + (areal ? 1 : 0);
@@ -334,9 +393,28 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
     set_error("gemm: too many tiles");
     return RENO_EINVAL;
   }
-  dim3 grid((unsigned)tiles, (unsigned)g.batch);
   const bool ak = (g.sAk == 1);
-  const bool bk = (g.sBk == 1) && !(g.sBn == 1 && g.K == 1);
+  const bool bk = (g.sBk == 1);
+  // split K when the tile grid cannot fill the chip and scratch is given
+  int nsplit = 1, kchunk = g.K > 0 ? g.K : 1;
+  split_plan(g.M, g.N, g.K, g.batch, &nsplit, &kchunk);
+  const size_t need = nsplit > 1 ? (size_t)nsplit * g.batch * g.M * g.N * elem_bytes(g.prec) : 0;
+  if (nsplit > 1 && (!g.part || g.part_bytes < need)) {
+    nsplit = 1;
+    kchunk = g.K > 0 ? g.K : 1;
+  }
+  GemmCall k = g;
+  k.kchunk = kchunk;
+  if (nsplit > 1) {
+    // partial sums [split][batch][M][N] in the compute precision, overwritten
+    k.C = g.part;
+    k.c_kind = g.prec == RENO_PREC_FP64 ? ST_Z128 : ST_C64;
+    k.sCm = g.N;
+    k.sCn = 1;
+    k.sCb = (int64_t)g.M * g.N;
+    k.beta = 0;
+  }
+  dim3 grid((unsigned)tiles, (unsigned)g.batch, (unsigned)nsplit);
   // K == 0 needs no special case: the kernel stores its zero accumulators.
   ProfRec rec{nullptr, nullptr, 0.0, 0};
   bool prof = false;
@@ -346,22 +424,47 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
       rec.a = pooled_event();
       rec.b = pooled_event();
       prof = rec.a && rec.b;
+      if (prof && g_exec) {
+        k.exec = g_exec + (g_prof.size() % kExecSlots);
+      }
     }
   }
   if (prof) (void)hipEventRecord(rec.a, stream);
   if (g.prec == RENO_PREC_FP64)
-    launch_r<double>(g, grid, stream, ak, bk, areal);
+    launch_r<double>(k, grid, stream, ak, bk, areal);
   else
-    launch_r<float>(g, grid, stream, ak, bk, areal);
+    launch_r<float>(k, grid, stream, ak, bk, areal);
   RENO_CHECK_LAUNCH();
+  // the timing bracket covers the matrix-core kernel only (the split-K
+  // reduction is a separate, bandwidth-bound kernel)
+  if (prof) (void)hipEventRecord(rec.b, stream);
+  if (nsplit > 1) {
+    const int64_t total = (int64_t)g.batch * g.M * g.N;
+    const int blocks = (int)((total + 255) / 256 < 8192 ? (total + 255) / 256 : 8192);
+    if (g.prec == RENO_PREC_FP64)
+      hipLaunchKernelGGL(k_split_reduce<double>, dim3(blocks), dim3(256), 0, stream,
+                         (const double2*)g.part, nsplit, g.batch, g.M, g.N, g.C, g.c_kind, g.sCm,
+                         g.sCb, g.beta, g.skip);
+    else
+      hipLaunchKernelGGL(k_split_reduce<float>, dim3(blocks), dim3(256), 0, stream,
+                         (const float2*)g.part, nsplit, g.batch, g.M, g.N, g.C, g.c_kind, g.sCm,
+                         g.sCb, g.beta, g.skip);
+    RENO_CHECK_LAUNCH();
+  }
   if (prof) {
-    (void)hipEventRecord(rec.b, stream);
     rec.flop = (areal ? 4.0 : 8.0) * (double)g.M * (double)g.N * (double)g.K * (double)g.batch;
     rec.kind = kind_index(g.prec, ak, bk, areal);
     std::lock_guard<std::mutex> lk(g_prof_mu);
+    rec.slot = k.exec ? (int)(k.exec - g_exec) : -1;
     g_prof.push_back(rec);
   }
   return RENO_OK;
+}
+
+size_t splitk_bytes(int prec, int M, int N, int K, int batch) {
+  int nsplit = 1, kchunk = 1;
+  split_plan(M, N, K, batch, &nsplit, &kchunk);
+  return nsplit > 1 ? (size_t)nsplit * batch * M * N * elem_bytes(prec) : 0;
 }
 
 int prof_enable(int on) {
@@ -373,6 +476,10 @@ int prof_enable(int on) {
       g_event_pool.push_back(r.b);
     }
     g_prof.clear();
+    if (!g_exec) RENO_HIP_CHECK(hipMalloc((void**)&g_exec, kExecSlots * sizeof(int)));
+    RENO_HIP_CHECK(hipDeviceSynchronize());
+    RENO_HIP_CHECK(hipMemset(g_exec, 0, kExecSlots * sizeof(int)));
+    RENO_HIP_CHECK(hipDeviceSynchronize());
   }
   return RENO_OK;
 }
@@ -386,8 +493,16 @@ int prof_collect(int nkinds, double* ms, double* flop, int* count) {
     flop[k] = 0.0;
     count[k] = 0;
   }
+  std::vector<int> executed;
+  if (g_exec && !g_prof.empty()) {
+    RENO_HIP_CHECK(hipDeviceSynchronize());
+    const size_t n = g_prof.size() < kExecSlots ? g_prof.size() : kExecSlots;
+    executed.resize(n);
+    RENO_HIP_CHECK(hipMemcpy(executed.data(), g_exec, n * sizeof(int), hipMemcpyDeviceToHost));
+  }
   for (auto& r : g_prof) {
     RENO_HIP_CHECK(hipEventSynchronize(r.b));
+    if (r.slot >= 0 && (size_t)r.slot < executed.size() && !executed[r.slot]) continue;
     float t = 0.f;
     RENO_HIP_CHECK(hipEventElapsedTime(&t, r.a, r.b));
     if (r.kind < nkinds) {

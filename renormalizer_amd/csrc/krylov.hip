@@ -2,12 +2,16 @@
 //
 // The three-term recurrence runs on the device in FP64 with deterministic
 // two-pass reductions (per-block partial sums, then one block folds them in a
-// fixed order), so repeated calls give bit-identical results.  The host reads
-// back alpha_j and beta_j once per iteration (the reference does the same:
-// krylov.py:57,71 are device->host syncs), solves the small symmetric
-// tridiagonal eigenproblem itself (cyclic Jacobi, FP64) and, at every even
-// j > 3, forms the Krylov combination on the device while counting elements
-// that violate allclose(res, new_res) in the same pass (krylov.py:76-81).
+// fixed order), so repeated calls give bit-identical results.
+//
+// The host keeps the reference's control flow (breakdown test, full-space
+// exit, allclose test at every even j > 3) but never idles the device on it:
+// iteration j+1 is queued before the host waits for alpha_j / beta_j, and a
+// device-side stop flag - raised by the breakdown test or by a converged
+// allclose check - turns every kernel of an iteration queued past the end into
+// a no-op.  Only the allclose checks need host values (the small tridiagonal
+// eigenproblem is solved on the host by cyclic Jacobi in FP64), so the device
+// waits for the host once per two iterations, for a few microseconds.
 #include <math.h>
 #include <string.h>
 
@@ -20,6 +24,7 @@ namespace reno {
 
 constexpr int kRedThreads = 256;
 constexpr int kMaxPartials = 1024;
+constexpr int kMaxKrylov = 2048;
 
 static int n_partials(int64_t n) {
   int64_t b = (n + kRedThreads * 4 - 1) / (kRedThreads * 4);
@@ -42,7 +47,9 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // partial[b] = sum over this block's grid-stride share of Re(conj(w) v)
 __global__ __launch_bounds__(kRedThreads) void k_dot_re(const double2* __restrict__ w,
                                                         const double2* __restrict__ v,
-                                                        int64_t n, double* partial) {
+                                                        int64_t n, double* partial,
+                                                        const int* stop) {
+  if (stop && *stop) return;
   __shared__ double sh[kRedThreads];
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
@@ -72,7 +79,8 @@ __global__ __launch_bounds__(kRedThreads) void k_norm2(const double2* __restrict
 __global__ __launch_bounds__(kRedThreads) void k_recur_norm2(
     double2* __restrict__ w, const double2* __restrict__ vj, const double2* __restrict__ vjm1,
     const double* __restrict__ alpha_j, const double* __restrict__ beta_jm1, int64_t n,
-    double* partial) {
+    double* partial, const int* stop) {
+  if (stop && *stop) return;
   __shared__ double sh[kRedThreads];
   const double al = *alpha_j;
   const double be = vjm1 ? *beta_jm1 : 0.0;
@@ -96,27 +104,37 @@ __global__ __launch_bounds__(kRedThreads) void k_recur_norm2(
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// out[0] = sum(partial) (mode 0) or sqrt(sum) (mode 1), fixed order
+// Fold the partials in a fixed order.
+// mode 0: out = sum                         (alpha_j)
+// mode 1: out = sqrt(sum); raise the stop flag when out < thr   (beta_j, breakdown)
+// mode 2: out = sum; raise the stop flag when sum == 0           (allclose violations)
+// mode 3: out = sqrt(sum); clear the stop flag                   (|v0|, start of a call)
 __global__ __launch_bounds__(kRedThreads) void k_fold(const double* __restrict__ partial,
-                                                      int np, double* out, int mode) {
+                                                      int np, double* out, int mode, double thr,
+                                                      int* stop) {
+  if (mode != 3 && stop && *stop) return;
   __shared__ double sh[kRedThreads];
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += kRedThreads) acc += partial[i];
   const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) out[0] = mode ? sqrt(s) : s;
+  if (threadIdx.x == 0) {
+    const double v = (mode == 1 || mode == 3) ? sqrt(s) : s;
+    out[0] = v;
+    if (mode == 1 && v < thr) *stop = 1;
+    if (mode == 2 && v == 0.0) *stop = 1;
+    if (mode == 3) *stop = 0;
+  }
 }
 
-// dst = src / (*den) or src / den_host when den == nullptr
+// dst = src / (*den)
 __global__ void k_div(const double2* __restrict__ src, double2* __restrict__ dst,
-                      const double* __restrict__ den, double den_host, int64_t n) {
-  const double q = den ? *den : den_host;
+                      const double* __restrict__ den, int64_t n, const int* stop) {
+  if (stop && *stop) return;
+  const double q = *den;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const double2 a = src[i];
-    double2 r;
-    r.x = a.x / q;
-    r.y = a.y / q;
-    dst[i] = r;
+    dst[i] = make_double2(a.x / q, a.y / q);
   }
 }
 
@@ -142,10 +160,7 @@ __global__ __launch_bounds__(kRedThreads) void k_combine(const double2* __restri
       xr += v.x * c.x - v.y * c.y;
       xi += v.x * c.y + v.y * c.x;
     }
-    double2 r;
-    r.x = xr;
-    r.y = xi;
-    res_new[i] = r;
+    res_new[i] = make_double2(xr, xi);
     if (res_old) {
       const double2 o = res_old[i];
       const double dr = o.x - xr, di = o.y - xi;
@@ -229,8 +244,8 @@ static void tridiag_expm_coef(int m, const double* alpha, const double* beta, do
 }
 
 struct KrylovLayout {
-  size_t hop_wl, hop_t1, hop_t2;
-  size_t off_wl, off_t1, off_t2, off_V, off_w, off_r0, off_r1, off_part, off_scal, off_coef;
+  size_t off_wl, off_t1, off_t2, off_split, off_V, off_w, off_r0, off_r1, off_part, off_scal, off_coef;
+  size_t split_bytes;
   size_t total;
 };
 
@@ -243,6 +258,8 @@ static KrylovLayout krylov_layout(const reno_hop_desc* op, int kcap) {
   k.off_wl = o; o += h.wl_bytes;
   k.off_t1 = o; o += h.t1_bytes;
   k.off_t2 = o; o += h.t2_bytes;
+  k.off_split = o; o += h.part_bytes;
+  k.split_bytes = h.part_bytes;
   k.off_V = o; o += vb * (size_t)kcap;
   k.off_w = o; o += vb;
   k.off_r0 = o; o += vb;
@@ -254,24 +271,36 @@ static KrylovLayout krylov_layout(const reno_hop_desc* op, int kcap) {
   return k;
 }
 
-// Pinned host staging for scalars read back every iteration (per thread).
-struct Pinned {
-  double* p = nullptr;
-  ~Pinned() {
-    if (p) (void)hipHostFree(p);
+// Per-thread host staging: pinned scalars (alpha_j, beta_j, stop flag and
+// allclose count per iteration; Krylov coefficients) and a pool of events.
+struct HostStage {
+  double* pin = nullptr;  // [2*j] alpha_j, [2*j+1] beta_j, then counts, then coefs
+  std::vector<hipEvent_t> events;
+  ~HostStage() {
+    if (pin) (void)hipHostFree(pin);
+    for (auto e : events) (void)hipEventDestroy(e);
   }
 };
-static thread_local Pinned t_pinned;
-static constexpr int kPinnedDoubles = 8192;
+static thread_local HostStage t_stage;
+static constexpr int kPinCoef = 2 * kMaxKrylov + 64;                 // coefficient area (doubles)
+static constexpr int kPinDoubles = kPinCoef + 2 * kMaxKrylov + 64;  // total
 
 static double* pinned_buf() {
-  if (!t_pinned.p) {
-    if (hipHostMalloc((void**)&t_pinned.p, kPinnedDoubles * sizeof(double),
-                      hipHostMallocDefault) != hipSuccess) {
-      t_pinned.p = nullptr;
-    }
+  if (!t_stage.pin) {
+    if (hipHostMalloc((void**)&t_stage.pin, kPinDoubles * sizeof(double), hipHostMallocDefault) !=
+        hipSuccess)
+      t_stage.pin = nullptr;
   }
-  return t_pinned.p;
+  return t_stage.pin;
+}
+
+static hipEvent_t stage_event(size_t i) {
+  while (t_stage.events.size() <= i) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    t_stage.events.push_back(e);
+  }
+  return t_stage.events[i];
 }
 
 }  // namespace reno
@@ -293,9 +322,8 @@ extern "C" int reno_expm_krylov(const reno_hop_desc* op, double dt_re, double dt
   }
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = hop_vec_len(op);
-  // capacity of the caller's workspace in Krylov vectors
   int kcap = 2;
-  while (krylov_layout(op, kcap + 1).total <= ws_bytes && kcap < 4096) ++kcap;
+  while (kcap < kMaxKrylov && krylov_layout(op, kcap + 1).total <= ws_bytes) ++kcap;
   const KrylovLayout L = krylov_layout(op, kcap);
   if (!ws || L.total > ws_bytes) {
     if (needed) *needed = krylov_layout(op, 16).total;
@@ -311,109 +339,136 @@ extern "C" int reno_expm_krylov(const reno_hop_desc* op, double dt_re, double dt
   void* wl = base + L.off_wl;
   void* t1 = base + L.off_t1;
   void* t2 = base + L.off_t2;
+  void* split = base + L.off_split;
   double2* V = (double2*)(base + L.off_V);
   const size_t vstride = align_up((size_t)n * 16) / 16;
   double2* w = (double2*)(base + L.off_w);
   double2* res[2] = {(double2*)(base + L.off_r0), (double2*)(base + L.off_r1)};
   double* part = (double*)(base + L.off_part);
-  double* scal = (double*)(base + L.off_scal);  // [0]=nrm [1..] alpha, beta
+  double* scal = (double*)(base + L.off_scal);  // [0]=|v0| [1]=violations [4]=stop flag (int)
+  int* stop = (int*)(scal + 4);
   double* d_alpha = scal + 8;
   double* d_beta = scal + 8 + kcap;
   double2* d_coef = (double2*)(base + L.off_coef);
   const int np = n_partials(n);
   const int ew_blocks = (int)((n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048);
+  const double thr = 100.0 * (double)n * 2.220446049250313e-16;  // krylov.py:72
 
   if (op->nsite == 1) RENO_TRY(permute_w(op->prec, op->W, wl, op->Dl, op->d, op->Dr, s));
 
-  // normalise the starting vector (krylov.py:42-45)
+  // |v0| and V[0] = v0 / |v0| (krylov.py:42-45); clears the stop flag
   hipLaunchKernelGGL(k_norm2, dim3(np), dim3(kRedThreads), 0, s, (const double2*)v0, n, part);
-  hipLaunchKernelGGL(k_fold, dim3(1), dim3(kRedThreads), 0, s, part, np, scal, 1);
-  RENO_CHECK_LAUNCH();
-  RENO_HIP_CHECK(hipMemcpyAsync(pin, scal, sizeof(double), hipMemcpyDeviceToHost, s));
-  RENO_HIP_CHECK(hipStreamSynchronize(s));
-  const double nrmv = pin[0];
-  if (!(nrmv > 0.0)) {
-    set_error("expm_krylov: zero starting vector (krylov.py:44 assert)");
-    return RENO_EINVAL;
-  }
+  hipLaunchKernelGGL(k_fold, dim3(1), dim3(kRedThreads), 0, s, (const double*)part, np, scal, 3,
+                     0.0, stop);
   hipLaunchKernelGGL(k_div, dim3(ew_blocks), dim3(256), 0, s, (const double2*)v0, V,
-                     (const double*)nullptr, nrmv, n);
+                     (const double*)scal, n, (const int*)nullptr);
   RENO_CHECK_LAUNCH();
+  RENO_HIP_CHECK(hipMemcpyAsync(pin + kPinCoef - 8, scal, sizeof(double), hipMemcpyDeviceToHost, s));
+
+  // queue iteration j: matvec, alpha_j, and (j < n-1) the recurrence, beta_j
+  // and V[j+1]; then stage alpha_j, beta_j and the stop flag to the host.
+  auto queue_iter = [&](int64_t j) -> int {
+    double2* vj = V + (size_t)j * vstride;
+    RENO_TRY(hop_apply_core(op, wl, vj, w, t1, t2, s, stop, split, L.split_bytes));
+    hipLaunchKernelGGL(k_dot_re, dim3(np), dim3(kRedThreads), 0, s, (const double2*)w,
+                       (const double2*)vj, n, part, (const int*)stop);
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(kRedThreads), 0, s, (const double*)part, np,
+                       d_alpha + j, 0, 0.0, stop);
+    if (j < n - 1) {
+      hipLaunchKernelGGL(k_recur_norm2, dim3(np), dim3(kRedThreads), 0, s, w, (const double2*)vj,
+                         j > 0 ? (const double2*)(vj - vstride) : (const double2*)nullptr,
+                         (const double*)(d_alpha + j), (const double*)(j > 0 ? d_beta + j - 1 : d_beta),
+                         n, part, (const int*)stop);
+      hipLaunchKernelGGL(k_fold, dim3(1), dim3(kRedThreads), 0, s, (const double*)part, np,
+                         d_beta + j, 1, thr, stop);
+      if (j + 1 < kcap)
+        hipLaunchKernelGGL(k_div, dim3(ew_blocks), dim3(256), 0, s, (const double2*)w,
+                           V + (size_t)(j + 1) * vstride, (const double*)(d_beta + j), n,
+                           (const int*)stop);
+    }
+    RENO_CHECK_LAUNCH();
+    RENO_HIP_CHECK(hipMemcpyAsync(pin + 2 * j, d_alpha + j, sizeof(double), hipMemcpyDeviceToHost, s));
+    if (j < n - 1)
+      RENO_HIP_CHECK(hipMemcpyAsync(pin + 2 * j + 1, d_beta + j, sizeof(double), hipMemcpyDeviceToHost, s));
+    hipEvent_t e = stage_event((size_t)j);
+    if (!e) {
+      set_error("expm_krylov: event creation failed");
+      return RENO_EHIP;
+    }
+    RENO_HIP_CHECK(hipEventRecord(e, s));
+    return RENO_OK;
+  };
 
   std::vector<double> alpha, beta;
   std::vector<double2> coef;
   int cur = 0;
   bool have_res = false;
-  const double eps = 2.220446049250313e-16;
+  double nrmv = -1.0;
 
-  auto finish = [&](int m, double2* dst) -> int {
+  // the combination sum_i coef_i V[i] for m vectors; with check, also count
+  // allclose violations against res[cur] and raise the stop flag when none
+  auto queue_combine = [&](int m, double2* dst, bool check) -> int {
     tridiag_expm_coef(m, alpha.data(), beta.data(), nrmv, dt_re, dt_im, coef);
-    memcpy(pin + 4096, coef.data(), sizeof(double2) * m);
-    RENO_HIP_CHECK(hipMemcpyAsync(d_coef, pin + 4096, sizeof(double2) * m,
+    memcpy(pin + kPinCoef, coef.data(), sizeof(double2) * m);
+    RENO_HIP_CHECK(hipMemcpyAsync(d_coef, pin + kPinCoef, sizeof(double2) * m,
                                   hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_combine, dim3(np), dim3(kRedThreads), 0, s, (const double2*)V,
-                       n, (int64_t)vstride, m, (const double2*)d_coef, (const double2*)nullptr, dst,
-                       (double*)nullptr);
+    hipLaunchKernelGGL(k_combine, dim3(np), dim3(kRedThreads), 0, s, (const double2*)V, n,
+                       (int64_t)vstride, m, (const double2*)d_coef,
+                       check ? (const double2*)res[cur] : (const double2*)nullptr, dst,
+                       check ? part : (double*)nullptr);
+    if (check)
+      hipLaunchKernelGGL(k_fold, dim3(1), dim3(kRedThreads), 0, s, (const double*)part, np,
+                         scal + 1, 2, 0.0, stop);
     RENO_CHECK_LAUNCH();
-    // the pinned coefficients must outlive the copy
-    RENO_HIP_CHECK(hipStreamSynchronize(s));
     return RENO_OK;
   };
 
+  RENO_TRY(queue_iter(0));
   for (int64_t j = 0; j < n; ++j) {
-    double2* vj = V + (size_t)j * vstride;
-    RENO_TRY(hop_apply_core(op, wl, vj, w, t1, t2, s));
-    hipLaunchKernelGGL(k_dot_re, dim3(np), dim3(kRedThreads), 0, s, (const double2*)w,
-                       (const double2*)vj, n, part);
-    hipLaunchKernelGGL(k_fold, dim3(1), dim3(kRedThreads), 0, s, part, np, d_alpha + j, 0);
-    RENO_CHECK_LAUNCH();
-    if (j == n - 1) {
-      // Krylov space equals the full space (krylov.py:59-61)
-      RENO_HIP_CHECK(hipMemcpyAsync(pin, d_alpha + j, sizeof(double), hipMemcpyDeviceToHost, s));
-      RENO_HIP_CHECK(hipStreamSynchronize(s));
-      alpha.push_back(pin[0]);
-      RENO_TRY(finish((int)j + 1, (double2*)out));
-      *n_matvec = (int)j + 1;
-      return RENO_OK;
-    }
-    hipLaunchKernelGGL(k_recur_norm2, dim3(np), dim3(kRedThreads), 0, s, w,
-                       (const double2*)vj,
-                       j > 0 ? (const double2*)(V + (size_t)(j - 1) * vstride) : (const double2*)nullptr,
-                       (const double*)(d_alpha + j), (const double*)(j > 0 ? d_beta + j - 1 : d_beta),
-                       n, part);
-    hipLaunchKernelGGL(k_fold, dim3(1), dim3(kRedThreads), 0, s, part, np, d_beta + j, 1);
-    RENO_CHECK_LAUNCH();
-    RENO_HIP_CHECK(hipMemcpyAsync(pin, d_alpha + j, sizeof(double), hipMemcpyDeviceToHost, s));
-    RENO_HIP_CHECK(hipMemcpyAsync(pin + 1, d_beta + j, sizeof(double), hipMemcpyDeviceToHost, s));
-    RENO_HIP_CHECK(hipStreamSynchronize(s));
-    alpha.push_back(pin[0]);
-    beta.push_back(pin[1]);
-    if (beta[j] < 100.0 * (double)n * eps) {
-      // Lanczos breakdown (krylov.py:72-74)
-      RENO_TRY(finish((int)j + 1, (double2*)out));
-      *n_matvec = (int)j + 1;
-      return RENO_OK;
-    }
-    if (j > 3 && j % 2 == 0) {
-      const int m = (int)j + 1;
-      tridiag_expm_coef(m, alpha.data(), beta.data(), nrmv, dt_re, dt_im, coef);
-      memcpy(pin + 4096, coef.data(), sizeof(double2) * m);
-      RENO_HIP_CHECK(hipMemcpyAsync(d_coef, pin + 4096, sizeof(double2) * m,
-                                    hipMemcpyHostToDevice, s));
-      double2* rnew = res[cur ^ 1];
-      hipLaunchKernelGGL(k_combine, dim3(np), dim3(kRedThreads), 0, s, (const double2*)V, n,
-                         (int64_t)vstride, m, (const double2*)d_coef,
-                         have_res ? (const double2*)res[cur] : (const double2*)nullptr, rnew,
-                         have_res ? part : (double*)nullptr);
-      RENO_CHECK_LAUNCH();
-      if (have_res) {
-        hipLaunchKernelGGL(k_fold, dim3(1), dim3(kRedThreads), 0, s, part, np, scal + 1, 0);
-        RENO_CHECK_LAUNCH();
-        RENO_HIP_CHECK(hipMemcpyAsync(pin + 2, scal + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    const bool last = (j == n - 1);
+    const bool check_pt = (j > 3 && j % 2 == 0 && !last);
+    // keep the device busy: queue the next iteration now unless this one
+    // needs a host decision first (allclose checks)
+    if (!check_pt && !last && j + 1 < kcap) RENO_TRY(queue_iter(j + 1));
+    RENO_HIP_CHECK(hipEventSynchronize(stage_event((size_t)j)));
+    if (nrmv < 0) {
+      nrmv = pin[kPinCoef - 8];
+      if (!(nrmv > 0.0)) {
+        set_error("expm_krylov: zero starting vector (krylov.py:44 assert)");
+        return RENO_EINVAL;
       }
+    }
+    alpha.push_back(pin[2 * j]);
+    if (last) {
+      // Krylov space equals the full space (krylov.py:59-61)
+      RENO_TRY(queue_combine((int)j + 1, (double2*)out, false));
+      RENO_HIP_CHECK(hipStreamSynchronize(s));  // pinned coefficients in flight
+      *n_matvec = (int)j + 1;
+      return RENO_OK;
+    }
+    beta.push_back(pin[2 * j + 1]);
+    if (beta[j] < thr) {
+      // Lanczos breakdown (krylov.py:72-74); the queued iteration j+1 was a no-op
+      RENO_TRY(queue_combine((int)j + 1, (double2*)out, false));
       RENO_HIP_CHECK(hipStreamSynchronize(s));
-      if (have_res && pin[2] == 0.0) {
-        RENO_HIP_CHECK(hipMemcpyAsync(out, rnew, (size_t)n * 16, hipMemcpyDeviceToDevice, s));
+      *n_matvec = (int)j + 1;
+      return RENO_OK;
+    }
+    if (check_pt) {
+      const int m = (int)j + 1;
+      RENO_TRY(queue_combine(m, res[cur ^ 1], have_res));
+      hipEvent_t ec = stage_event((size_t)kMaxKrylov);
+      if (have_res) {
+        RENO_HIP_CHECK(hipMemcpyAsync(pin + kPinCoef - 4, scal + 1, sizeof(double),
+                                      hipMemcpyDeviceToHost, s));
+      }
+      RENO_HIP_CHECK(hipEventRecord(ec, s));
+      // the next iteration goes behind the check; it is cancelled on device
+      // if the check converged
+      if (j + 1 < kcap) RENO_TRY(queue_iter(j + 1));
+      RENO_HIP_CHECK(hipEventSynchronize(ec));
+      if (have_res && pin[kPinCoef - 4] == 0.0) {
+        RENO_HIP_CHECK(hipMemcpyAsync(out, res[cur ^ 1], (size_t)n * 16, hipMemcpyDeviceToDevice, s));
         *n_matvec = (int)j + 1;
         return RENO_OK;
       }
@@ -422,12 +477,10 @@ extern "C" int reno_expm_krylov(const reno_hop_desc* op, double dt_re, double dt
     }
     if (j + 1 >= kcap) {
       if (needed) *needed = krylov_layout(op, 2 * kcap).total;
+      RENO_HIP_CHECK(hipStreamSynchronize(s));
       set_error("expm_krylov: %d Krylov vectors do not fit the workspace", kcap);
       return RENO_ENOSPACE;
     }
-    hipLaunchKernelGGL(k_div, dim3(ew_blocks), dim3(256), 0, s, (const double2*)w,
-                       V + (size_t)(j + 1) * vstride, (const double*)(d_beta + j), 0.0, n);
-    RENO_CHECK_LAUNCH();
   }
   set_error("expm_krylov: unreachable");
   return RENO_EINVAL;

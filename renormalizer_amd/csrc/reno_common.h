@@ -74,9 +74,22 @@ struct GemmCall {
   int c_kind;
   int64_t sCm, sCn, sCb;
   int beta;
+  // optional device flag: when *skip != 0 the launch does nothing (used by
+  // the Lanczos loop to cancel speculatively queued iterations)
+  const int* skip;
+  // optional scratch for split-K partial sums (see splitk_bytes); without it
+  // the GEMM runs unsplit
+  void* part;
+  size_t part_bytes;
+  // set by launch_gemm: K range per blockIdx.z; launch-executed marker used
+  // by the timing hook (so launches cancelled through `skip` are not counted)
+  int kchunk;
+  int* exec;
 };
 
 int launch_gemm(const GemmCall& g, hipStream_t stream);
+// bytes of split-K scratch launch_gemm would use for this shape (0: no split)
+size_t splitk_bytes(int prec, int M, int N, int K, int batch);
 
 inline size_t align_up(size_t x, size_t a = 256) { return (x + a - 1) / a * a; }
 

@@ -301,7 +301,10 @@ class Mps(MatrixProduct):
         cache = self._env_cache
         if cache is not None and cache[0] is mpo and cache[1] == prec and not mps.to_right \
                 and mps.qnidx == n - 1:
+            # the sweep below rewrites these environments in place: take them
+            # away from self so a second evolution of self rebuilds its own
             environ = cache[2]
+            self._env_cache = None
         else:
             # only the left environments are read before being rebuilt
             environ = Environ(mps._mp, wt, "L" if not mps.to_right else "R", prec=prec)

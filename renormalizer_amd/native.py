@@ -310,13 +310,11 @@ def block_qr(system, coef, blocks, v_transposed=False):
     if len(blocks) == 0:
         raise ValueError("Invalid quantum number")
     dev = coef.device
-    idx = []
     cblocks = (QnBlock * len(blocks))()
     kt = 0
     for i, (r, c) in enumerate(blocks):
-        rt = torch.as_tensor(np.ascontiguousarray(r, dtype=np.int64)).to(dev, non_blocking=False)
-        ct = torch.as_tensor(np.ascontiguousarray(c, dtype=np.int64)).to(dev, non_blocking=False)
-        idx.append((rt, ct))
+        rt = _device_index(r, dev)
+        ct = _device_index(c, dev)
         cblocks[i] = QnBlock(len(r), len(c), rt.data_ptr(), ct.data_ptr())
         kt += min(len(r), len(c))
     u = torch.empty((rows, kt), dtype=torch.complex128, device=dev)
@@ -327,10 +325,26 @@ def block_qr(system, coef, blocks, v_transposed=False):
     sysc = {"L": 0, "R": 1}[system]
     check(l.reno_block_qr(sysc, rows, cols, _ptr(coef), len(blocks), cblocks, _ptr(u), _ptr(v),
                           int(bool(v_transposed)), _ptr(ws), nbytes, _stream()), "reno_block_qr")
-    # keep the index tensors alive until the kernels that read them have run
-    torch.cuda.current_stream().synchronize()
-    del idx
     return u, v
+
+
+_INDEX_CACHE = {}
+_INDEX_CACHE_MAX = 4096
+
+
+def _device_index(a, dev):
+    """Device copy of a host int64 index array, cached by content.  Cached
+    tensors stay alive, and an evicted one is only reused by allocations that
+    are stream-ordered after the kernels reading it."""
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    key = (str(dev), a.tobytes())
+    t = _INDEX_CACHE.get(key)
+    if t is None:
+        if len(_INDEX_CACHE) >= _INDEX_CACHE_MAX:
+            _INDEX_CACHE.pop(next(iter(_INDEX_CACHE)))
+        t = torch.from_numpy(a).to(dev)
+        _INDEX_CACHE[key] = t
+    return t
 
 
 def env_close(X, Y):

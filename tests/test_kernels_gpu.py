@@ -74,6 +74,7 @@ SHAPES = [
     (64, 2, 64, 5, 4),     # Holstein electron site
     (64, 10, 64, 4, 5),    # Holstein phonon site
     (33, 7, 45, 2, 3),     # ragged
+    (40, 6, 100, 3, 7),    # ragged split-K chunks (long K, few output tiles)
     (256, 10, 256, 3, 3),  # spin-boson M=256 full site
 ]
 
