@@ -90,6 +90,8 @@ def run_mode(mps0, mpo, precision, dt, warmup, steps, profile):
     matvecs = [sum(m.evolve_config.local_steps) for m in history[1 + warmup:]]
     sz = [sigma_z(m) for m in history]
     env = mps._env_cache[2] if mps._env_cache else None
+    from renormalizer_amd import Mps
+    log(f"{precision}: steps redone after an exhausted Krylov budget so far: {Mps.krylov_redo_count}")
     return {"seconds": t1 - t0, "steps_per_s": steps / (t1 - t0), "sz": sz, "prof": prof,
             "matvecs_per_step": matvecs, "final": mps, "environ": env}
 

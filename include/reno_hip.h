@@ -115,6 +115,19 @@ int reno_expm_krylov(const reno_hop_desc* op, double dt_re, double dt_im,
                      const void* v0, void* out, int* n_matvec, size_t* needed,
                      void* ws, size_t ws_bytes, void* stream);
 
+/* ---- expm_krylov without host waits (same computation as
+ * reno_expm_krylov; the stopping rule is evaluated on the device) --------
+ * Queues at most max_matvec Lanczos iterations (the workspace must hold that
+ * many Krylov vectors: reno_krylov_workspace_bytes(op, max_matvec)) and the
+ * final combination into out, then returns without waiting.  info is a
+ * device int[2]: info[0] = matvecs used, info[1] = 1 when the reference's
+ * rule (allclose / breakdown / full space) ended the iteration, 0 when
+ * max_matvec iterations ran without it (out is then NOT the result: repeat
+ * with reno_expm_krylov), -1 for a zero starting vector. */
+int reno_expm_krylov_async(const reno_hop_desc* op, double dt_re, double dt_im,
+                           const void* v0, void* out, int max_matvec, int* info,
+                           void* ws, size_t ws_bytes, void* stream);
+
 /* ---- svd_qn QR / RQ branch (replaces svd_qn.py:99-240, QR=True,
  * full_matrices=False) ------------------------------------------------
  * coef is the complex128 (rows x cols) coefficient matrix.  For each block

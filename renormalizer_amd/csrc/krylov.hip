@@ -4,18 +4,22 @@
 // two-pass reductions (per-block partial sums, then one block folds them in a
 // fixed order), so repeated calls give bit-identical results.
 //
-// The host keeps the reference's control flow (breakdown test, full-space
-// exit, allclose test at every even j > 3) but never idles the device on it:
-// iteration j+1 is queued before the host waits for alpha_j / beta_j, and a
-// device-side stop flag - raised by the breakdown test or by a converged
-// allclose check - turns every kernel of an iteration queued past the end into
-// a no-op.  Only the allclose checks need host values (the small tridiagonal
-// eigenproblem is solved on the host by cyclic Jacobi in FP64), so the device
-// waits for the host once per two iterations, for a few microseconds.
+// The reference's control flow runs on the device too: the breakdown test
+// (beta_j < 100 n eps), the full-space exit, and at every even j > 3 the
+// small exponential expm(dt T) e_0 of the tridiagonal Lanczos matrix
+// followed by numpy.allclose(res, new_res) on the combined vector.  A
+// device-side state block records why and where the iteration ended, and a
+// stop flag turns every kernel queued past that point into a no-op, so the
+// host queues iterations ahead without waiting for any of them:
+//   * reno_expm_krylov queues chunks of iterations and reads the state once
+//     per chunk (normally once per call);
+//   * reno_expm_krylov_async queues up to max_matvec iterations plus the
+//     finalisation and returns at once; the outcome lands in a device info
+//     array that the caller reads whenever it likes.
 #include <math.h>
 #include <string.h>
 
-#include <complex>
+#include <algorithm>
 #include <vector>
 
 #include "contract.h"
@@ -24,7 +28,17 @@ namespace reno {
 
 constexpr int kRedThreads = 256;
 constexpr int kMaxPartials = 1024;
-constexpr int kMaxKrylov = 2048;
+constexpr int kMaxKrylov = 1024;  // 64 lanes x 16 entries in k_tri_coef
+
+// device state block (ints) at scal + 2
+enum {
+  KS_STOP = 0,     // nonzero: every later iteration kernel returns at once
+  KS_STATUS = 1,   // 0 running, 1 converged (allclose), 2 breakdown, 3 full space, -1 zero v0
+  KS_MFINAL = 2,   // Krylov vectors of the final combination (status 2, 3)
+  KS_RESIDX = 3,   // result buffer holding the converged vector (status 1)
+  KS_NMATVEC = 4,  // matvecs used (reference j + 1)
+  KS_INTS = 8
+};
 
 static int n_partials(int64_t n) {
   int64_t b = (n + kRedThreads * 4 - 1) / (kRedThreads * 4);
@@ -49,7 +63,7 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_re(const double2* __restric
                                                         const double2* __restrict__ v,
                                                         int64_t n, double* partial,
                                                         const int* stop) {
-  if (stop && *stop) return;
+  if (*stop) return;
   __shared__ double sh[kRedThreads];
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
@@ -75,12 +89,12 @@ __global__ __launch_bounds__(kRedThreads) void k_norm2(const double2* __restrict
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// w -= alpha v_j + beta v_{j-1}  (krylov.py:70), partial[b] = sum |w|^2
+// w -= (alpha v_j + beta v_{j-1})  (krylov.py:70), partial[b] = sum |w|^2
 __global__ __launch_bounds__(kRedThreads) void k_recur_norm2(
     double2* __restrict__ w, const double2* __restrict__ vj, const double2* __restrict__ vjm1,
     const double* __restrict__ alpha_j, const double* __restrict__ beta_jm1, int64_t n,
     double* partial, const int* stop) {
-  if (stop && *stop) return;
+  if (*stop) return;
   __shared__ double sh[kRedThreads];
   const double al = *alpha_j;
   const double be = vjm1 ? *beta_jm1 : 0.0;
@@ -104,32 +118,88 @@ __global__ __launch_bounds__(kRedThreads) void k_recur_norm2(
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// Fold the partials in a fixed order.
-// mode 0: out = sum                         (alpha_j)
-// mode 1: out = sqrt(sum); raise the stop flag when out < thr   (beta_j, breakdown)
-// mode 2: out = sum; raise the stop flag when sum == 0           (allclose violations)
-// mode 3: out = sqrt(sum); clear the stop flag                   (|v0|, start of a call)
-__global__ __launch_bounds__(kRedThreads) void k_fold(const double* __restrict__ partial,
-                                                      int np, double* out, int mode, double thr,
-                                                      int* stop) {
-  if (mode != 3 && stop && *stop) return;
+// Start of a call: scal[0] = |v0|, state cleared; a zero v0 (the reference's
+// assert, krylov.py:44) ends the call with status -1.
+__global__ __launch_bounds__(kRedThreads) void k_kry_init(const double* __restrict__ partial,
+                                                          int np, double* scal, int* st) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += kRedThreads) acc += partial[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x < KS_INTS) st[threadIdx.x] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt(s);
+    scal[0] = nrm;
+    if (!(nrm > 0.0)) {
+      st[KS_STOP] = 1;
+      st[KS_STATUS] = -1;
+    }
+  }
+}
+
+// alpha_j = sum of partials
+__global__ __launch_bounds__(kRedThreads) void k_fold_alpha(const double* __restrict__ partial,
+                                                            int np, double* out, const int* st) {
+  if (st[KS_STOP]) return;
+  __shared__ double sh[kRedThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += kRedThreads) acc += partial[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
+// beta_j = sqrt(sum); breakdown (krylov.py:72-74) when beta_j < thr: the
+// result is the combination of the j + 1 vectors built so far
+__global__ __launch_bounds__(kRedThreads) void k_fold_beta(const double* __restrict__ partial,
+                                                           int np, double* out, double thr, int j,
+                                                           int* st) {
+  if (st[KS_STOP]) return;
   __shared__ double sh[kRedThreads];
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += kRedThreads) acc += partial[i];
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) {
-    const double v = (mode == 1 || mode == 3) ? sqrt(s) : s;
+    const double v = sqrt(s);
     out[0] = v;
-    if (mode == 1 && v < thr) *stop = 1;
-    if (mode == 2 && v == 0.0) *stop = 1;
-    if (mode == 3) *stop = 0;
+    if (v < thr) {
+      st[KS_STOP] = 1;
+      st[KS_STATUS] = 2;
+      st[KS_MFINAL] = j + 1;
+      st[KS_NMATVEC] = j + 1;
+    }
+  }
+}
+
+// Krylov space equals the full space after iteration n-1 (krylov.py:59-61)
+__global__ void k_mark_full(int n, int* st) {
+  if (st[KS_STOP]) return;
+  st[KS_STOP] = 1;
+  st[KS_STATUS] = 3;
+  st[KS_MFINAL] = n;
+  st[KS_NMATVEC] = n;
+}
+
+// allclose check at iteration j: no violations -> converged, result in res[residx]
+__global__ __launch_bounds__(kRedThreads) void k_fold_check(const double* __restrict__ partial,
+                                                            int np, int j, int residx, int* st) {
+  if (st[KS_STOP]) return;
+  __shared__ double sh[kRedThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += kRedThreads) acc += partial[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0 && s == 0.0) {
+    st[KS_STOP] = 1;
+    st[KS_STATUS] = 1;
+    st[KS_RESIDX] = residx;
+    st[KS_NMATVEC] = j + 1;
   }
 }
 
 // dst = src / (*den)
 __global__ void k_div(const double2* __restrict__ src, double2* __restrict__ dst,
                       const double* __restrict__ den, int64_t n, const int* stop) {
-  if (stop && *stop) return;
+  if (*stop) return;
   const double q = *den;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -138,14 +208,111 @@ __global__ void k_div(const double2* __restrict__ src, double2* __restrict__ dst
   }
 }
 
-// res_new[i] = sum_j coef[j] V[j][i]; if res_old: partial count of elements
-// violating |res_old - res_new| <= atol + rtol |res_new| (numpy.allclose(res_old, res_new)).
+__device__ __forceinline__ double wave_min(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// coef = |v0| expm(dt T) e_0 for the m x m tridiagonal Lanczos matrix T
+// (krylov.py:15-24 forms the same vector from eigh_tridiagonal).  Computed
+// as exp(dt c) [Taylor_K(h (T - c I))]^q e_0 with c the centre and r the
+// half-width of T's Gershgorin interval, q = ceil(|dt| r / 4), h = dt / q:
+// every factor has ||h (T - c I)|| <= 4, and K = 34 terms leave a truncation
+// error below 4^35 / 35! ~ 1e-19.  One wavefront; lane l holds entries
+// l*R .. l*R+R-1; neighbours cross lanes by shuffles.  Fixed operation order.
+// final == 0: the check of iteration m-1 (skipped once stopped);
+// final == 1: end of the call, m = MFINAL, run only for status 2 / 3.
+template <int R>
+__global__ __launch_bounds__(64) void k_tri_coef(const double* __restrict__ alpha,
+                                                 const double* __restrict__ beta,
+                                                 const double* __restrict__ scal, double dt_re,
+                                                 double dt_im, int m_in, int final,
+                                                 const int* st, double2* __restrict__ coef) {
+  if (!final && st[KS_STOP]) return;
+  if (final && st[KS_STATUS] != 2 && st[KS_STATUS] != 3) return;
+  const int m = final ? st[KS_MFINAL] : m_in;
+  const int lane = threadIdx.x;
+  double a[R], bl[R], br[R];
+  double lo = 1e300, hi = -1e300;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = lane * R + r;
+    const bool in = i < m;
+    a[r] = in ? alpha[i] : 0.0;
+    bl[r] = (in && i > 0) ? beta[i - 1] : 0.0;
+    br[r] = (in && i + 1 < m) ? beta[i] : 0.0;
+    if (in) {
+      lo = fmin(lo, a[r] - fabs(bl[r]) - fabs(br[r]));
+      hi = fmax(hi, a[r] + fabs(bl[r]) + fabs(br[r]));
+    }
+  }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  const double c = 0.5 * (lo + hi);
+  const double rad = 0.5 * (hi - lo);
+#pragma unroll
+  for (int r = 0; r < R; ++r) a[r] -= c;
+  const double adt = sqrt(dt_re * dt_re + dt_im * dt_im);
+  int q = (int)ceil(adt * rad / 4.0);
+  if (q < 1) q = 1;
+  const double hr = dt_re / q, hi_ = dt_im / q;
+  double yr[R], yi[R], xr[R], xi[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    yr[r] = (lane == 0 && r == 0) ? scal[0] : 0.0;
+    yi[r] = 0.0;
+  }
+  for (int step = 0; step < q; ++step) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      xr[r] = yr[r];
+      xi[r] = yi[r];
+    }
+    for (int k = 1; k <= 34; ++k) {
+      // neighbours across lanes: the entry before this lane's first and the
+      // one after its last (zero-weighted at the ends via bl / br)
+      const double lr = __shfl_up(xr[R - 1], 1, 64), li = __shfl_up(xi[R - 1], 1, 64);
+      const double rr = __shfl_down(xr[0], 1, 64), ri = __shfl_down(xi[0], 1, 64);
+      const double fr = hr / (double)k, fi = hi_ / (double)k;
+      double pr = lr, pi = li;  // old x[r-1]
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double nr = r + 1 < R ? xr[r + 1] : rr;
+        const double ni = r + 1 < R ? xi[r + 1] : ri;
+        const double tr = a[r] * xr[r] + bl[r] * pr + br[r] * nr;
+        const double ti = a[r] * xi[r] + bl[r] * pi + br[r] * ni;
+        pr = xr[r];
+        pi = xi[r];
+        xr[r] = tr * fr - ti * fi;
+        xi[r] = tr * fi + ti * fr;
+        yr[r] += xr[r];
+        yi[r] += xi[r];
+      }
+    }
+  }
+  const double e = exp(dt_re * c);
+  const double pr = e * cos(dt_im * c), pi = e * sin(dt_im * c);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = lane * R + r;
+    if (i < m) coef[i] = make_double2(yr[r] * pr - yi[r] * pi, yr[r] * pi + yi[r] * pr);
+  }
+}
+
+// res_new[i] = sum_j coef[j] V[j][i] and the count of elements violating
+// |res_old - res_new| <= atol + rtol |res_new| (numpy.allclose(res_old,
+// res_new), krylov.py:76-81); res_old may be null (first check: no count).
 __global__ __launch_bounds__(kRedThreads) void k_combine(const double2* __restrict__ V,
                                                          int64_t n, int64_t vstride, int nv,
                                                          const double2* __restrict__ coef,
                                                          const double2* __restrict__ res_old,
                                                          double2* __restrict__ res_new,
-                                                         double* partial) {
+                                                         double* partial, const int* st) {
+  if (st[KS_STOP]) return;
   __shared__ double sh[kRedThreads];
   __shared__ double2 sc[64];
   for (int j = threadIdx.x; j < nv && j < 64; j += kRedThreads) sc[j] = coef[j];
@@ -169,82 +336,55 @@ __global__ __launch_bounds__(kRedThreads) void k_combine(const double2* __restri
       if (!(lhs <= rhs)) viol += 1.0;
     }
   }
-  if (partial) {
+  if (res_old) {
     const double s = block_sum(viol, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
   }
 }
 
-// ---------------------------------------------------------------------------
-// Host side: eigen-decomposition of the symmetric tridiagonal Lanczos matrix
-// by cyclic Jacobi rotations (FP64), then coef = U (nrm exp(dt w) U[0,:]).
-static void jacobi_eigh(int m, std::vector<double>& a, std::vector<double>& w,
-                        std::vector<double>& u) {
-  u.assign((size_t)m * m, 0.0);
-  for (int i = 0; i < m; ++i) u[(size_t)i * m + i] = 1.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, tot = 0.0;
-    for (int p = 0; p < m; ++p)
-      for (int q = 0; q < m; ++q) {
-        const double x = a[(size_t)p * m + q];
-        tot += x * x;
-        if (p != q) off += x * x;
+// End of a call: out = the converged combination (status 1) or the
+// combination of MFINAL vectors (breakdown / full space, coefficients from
+// the final k_tri_coef); nothing when the iteration budget ran out.
+__global__ __launch_bounds__(kRedThreads) void k_final_out(const double2* __restrict__ V,
+                                                           int64_t n, int64_t vstride,
+                                                           const double2* __restrict__ coef,
+                                                           const double2* __restrict__ res0,
+                                                           const double2* __restrict__ res1,
+                                                           double2* __restrict__ out,
+                                                           const int* st) {
+  const int status = st[KS_STATUS];
+  if (status == 1) {
+    const double2* src = st[KS_RESIDX] ? res1 : res0;
+    for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kRedThreads)
+      out[i] = src[i];
+  } else if (status == 2 || status == 3) {
+    const int nv = st[KS_MFINAL];
+    for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kRedThreads) {
+      double xr = 0.0, xi = 0.0;
+      for (int j = 0; j < nv; ++j) {
+        const double2 c = coef[j];
+        const double2 v = V[(int64_t)j * vstride + i];
+        xr += v.x * c.x - v.y * c.y;
+        xi += v.x * c.y + v.y * c.x;
       }
-    if (off <= 1e-34 * tot || off == 0.0) break;
-    for (int p = 0; p < m - 1; ++p)
-      for (int q = p + 1; q < m; ++q) {
-        const double apq = a[(size_t)p * m + q];
-        if (apq == 0.0) continue;
-        const double app = a[(size_t)p * m + p], aqq = a[(size_t)q * m + q];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(t * t + 1.0);
-        const double s = t * c;
-        for (int k = 0; k < m; ++k) {  // A <- A J
-          const double akp = a[(size_t)k * m + p], akq = a[(size_t)k * m + q];
-          a[(size_t)k * m + p] = c * akp - s * akq;
-          a[(size_t)k * m + q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < m; ++k) {  // A <- J^T A
-          const double apk = a[(size_t)p * m + k], aqk = a[(size_t)q * m + k];
-          a[(size_t)p * m + k] = c * apk - s * aqk;
-          a[(size_t)q * m + k] = s * apk + c * aqk;
-        }
-        for (int k = 0; k < m; ++k) {  // U <- U J
-          const double ukp = u[(size_t)k * m + p], ukq = u[(size_t)k * m + q];
-          u[(size_t)k * m + p] = c * ukp - s * ukq;
-          u[(size_t)k * m + q] = s * ukp + c * ukq;
-        }
-      }
+      out[i] = make_double2(xr, xi);
+    }
   }
-  w.resize(m);
-  for (int i = 0; i < m; ++i) w[i] = a[(size_t)i * m + i];
 }
 
-// coefficients of the Krylov vectors for expm(dt T) (nrm e_0), krylov.py:15-24
-static void tridiag_expm_coef(int m, const double* alpha, const double* beta, double nrm,
-                              double dt_re, double dt_im, std::vector<double2>& coef) {
-  std::vector<double> a((size_t)m * m, 0.0), w, u;
-  for (int i = 0; i < m; ++i) a[(size_t)i * m + i] = alpha[i];
-  for (int i = 0; i + 1 < m; ++i) {
-    a[(size_t)i * m + i + 1] = beta[i];
-    a[(size_t)(i + 1) * m + i] = beta[i];
-  }
-  jacobi_eigh(m, a, w, u);
-  std::vector<std::complex<double>> g(m);
-  const std::complex<double> dt(dt_re, dt_im);
-  for (int k = 0; k < m; ++k) g[k] = nrm * std::exp(dt * w[k]) * u[k];  // u[0*m+k] = U[0][k]
-  coef.resize(m);
-  for (int i = 0; i < m; ++i) {
-    std::complex<double> acc(0.0, 0.0);
-    for (int k = 0; k < m; ++k) acc += u[(size_t)i * m + k] * g[k];
-    coef[i].x = acc.real();
-    coef[i].y = acc.imag();
-  }
+// info[0] = matvecs used; info[1] = 1 finished by the reference's rule,
+// 0 iteration budget exhausted (out not written), -1 zero starting vector
+__global__ void k_info(const int* st, int queued, int* info) {
+  const int status = st[KS_STATUS];
+  info[0] = status == 0 ? queued : st[KS_NMATVEC];
+  info[1] = status == 0 ? 0 : (status < 0 ? -1 : 1);
 }
 
 struct KrylovLayout {
-  size_t off_wl, off_t1, off_t2, off_split, off_V, off_w, off_r0, off_r1, off_part, off_scal, off_coef;
+  size_t off_wl, off_t1, off_t2, off_split, off_V, off_w, off_r0, off_r1, off_part, off_scal,
+      off_coef;
   size_t split_bytes;
   size_t total;
 };
@@ -271,36 +411,166 @@ static KrylovLayout krylov_layout(const reno_hop_desc* op, int kcap) {
   return k;
 }
 
-// Per-thread host staging: pinned scalars (alpha_j, beta_j, stop flag and
-// allclose count per iteration; Krylov coefficients) and a pool of events.
+// Per-thread pinned host copy of the device state block (synchronous entry
+// point) and the event it waits on.
 struct HostStage {
-  double* pin = nullptr;  // [2*j] alpha_j, [2*j+1] beta_j, then counts, then coefs
-  std::vector<hipEvent_t> events;
+  int* pin = nullptr;
+  hipEvent_t ev = nullptr;
   ~HostStage() {
     if (pin) (void)hipHostFree(pin);
-    for (auto e : events) (void)hipEventDestroy(e);
+    if (ev) (void)hipEventDestroy(ev);
   }
 };
 static thread_local HostStage t_stage;
-static constexpr int kPinCoef = 2 * kMaxKrylov + 64;                 // coefficient area (doubles)
-static constexpr int kPinDoubles = kPinCoef + 2 * kMaxKrylov + 64;  // total
 
-static double* pinned_buf() {
-  if (!t_stage.pin) {
-    if (hipHostMalloc((void**)&t_stage.pin, kPinDoubles * sizeof(double), hipHostMallocDefault) !=
-        hipSuccess)
-      t_stage.pin = nullptr;
-  }
-  return t_stage.pin;
+// One call's device buffers and launch parameters.
+struct KrylovRun {
+  const reno_hop_desc* op;
+  hipStream_t s;
+  int64_t n;
+  int kcap;
+  KrylovLayout L;
+  void *wl, *t1, *t2, *split;
+  double2* V;
+  size_t vstride;
+  double2* w;
+  double2* res[2];
+  double* part;
+  double* scal;
+  int* st;
+  double* d_alpha;
+  double* d_beta;
+  double2* d_coef;
+  int np, ew_blocks;
+  double thr, dt_re, dt_im;
+};
+
+static int kcap_for(const reno_hop_desc* op, size_t ws_bytes) {
+  int kcap = 2;
+  while (kcap < kMaxKrylov && krylov_layout(op, kcap + 1).total <= ws_bytes) ++kcap;
+  return kcap;
 }
 
-static hipEvent_t stage_event(size_t i) {
-  while (t_stage.events.size() <= i) {
-    hipEvent_t e = nullptr;
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
-    t_stage.events.push_back(e);
+static void bind_run(KrylovRun& r, const reno_hop_desc* op, hipStream_t s, void* ws, int kcap,
+                     double dt_re, double dt_im) {
+  r.op = op;
+  r.s = s;
+  r.n = hop_vec_len(op);
+  r.kcap = kcap;
+  r.L = krylov_layout(op, kcap);
+  char* base = (char*)ws;
+  r.wl = base + r.L.off_wl;
+  r.t1 = base + r.L.off_t1;
+  r.t2 = base + r.L.off_t2;
+  r.split = base + r.L.off_split;
+  r.V = (double2*)(base + r.L.off_V);
+  r.vstride = align_up((size_t)r.n * 16) / 16;
+  r.w = (double2*)(base + r.L.off_w);
+  r.res[0] = (double2*)(base + r.L.off_r0);
+  r.res[1] = (double2*)(base + r.L.off_r1);
+  r.part = (double*)(base + r.L.off_part);
+  r.scal = (double*)(base + r.L.off_scal);  // [0] |v0|, [1] unused, [2..5] state ints
+  r.st = (int*)(r.scal + 2);
+  r.d_alpha = r.scal + 8;
+  r.d_beta = r.scal + 8 + kcap;
+  r.d_coef = (double2*)(base + r.L.off_coef);
+  r.np = n_partials(r.n);
+  r.ew_blocks = (int)((r.n + 255) / 256 < 2048 ? (r.n + 255) / 256 : 2048);
+  r.thr = 100.0 * (double)r.n * 2.220446049250313e-16;  // krylov.py:72
+  r.dt_re = dt_re;
+  r.dt_im = dt_im;
+}
+
+// |v0|, state reset, V[0] = v0 / |v0| (krylov.py:42-45)
+static int queue_start(KrylovRun& r, const void* v0) {
+  if (r.op->nsite == 1)
+    RENO_TRY(permute_w(r.op->prec, r.op->W, r.wl, r.op->Dl, r.op->d, r.op->Dr, r.s));
+  hipLaunchKernelGGL(k_norm2, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)v0, r.n,
+                     r.part);
+  hipLaunchKernelGGL(k_kry_init, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part, r.np,
+                     r.scal, r.st);
+  hipLaunchKernelGGL(k_div, dim3(r.ew_blocks), dim3(256), 0, r.s, (const double2*)v0, r.V,
+                     (const double*)r.scal, r.n, (const int*)r.st);
+  RENO_CHECK_LAUNCH();
+  return RENO_OK;
+}
+
+// k_tri_coef instantiation for up to m entries
+static void launch_tri_coef(KrylovRun& r, int m, int final) {
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(1), dim3(64), 0, r.s, (const double*)r.d_alpha,
+                       (const double*)r.d_beta, (const double*)r.scal, r.dt_re, r.dt_im, m, final,
+                       (const int*)r.st, r.d_coef);
+  };
+  if (m <= 64) go(k_tri_coef<1>);
+  else if (m <= 128) go(k_tri_coef<2>);
+  else if (m <= 256) go(k_tri_coef<4>);
+  else if (m <= 512) go(k_tri_coef<8>);
+  else go(k_tri_coef<16>);
+}
+
+// iteration j (krylov.py:55-81): matvec, alpha_j; unless j is the last
+// possible one: recurrence, beta_j with the breakdown test, V[j+1]; at even
+// j > 3 the combination and (from the second check on) the allclose test.
+static int queue_iter(KrylovRun& r, int64_t j) {
+  const int* stop = r.st + KS_STOP;
+  double2* vj = r.V + (size_t)j * r.vstride;
+  RENO_TRY(hop_apply_core(r.op, r.wl, vj, r.w, r.t1, r.t2, r.s, stop, r.split, r.L.split_bytes));
+  hipLaunchKernelGGL(k_dot_re, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.w,
+                     (const double2*)vj, r.n, r.part, stop);
+  hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part,
+                     r.np, r.d_alpha + j, (const int*)r.st);
+  if (j == r.n - 1) {
+    hipLaunchKernelGGL(k_mark_full, dim3(1), dim3(1), 0, r.s, (int)r.n, r.st);
+    RENO_CHECK_LAUNCH();
+    return RENO_OK;
   }
-  return t_stage.events[i];
+  hipLaunchKernelGGL(k_recur_norm2, dim3(r.np), dim3(kRedThreads), 0, r.s, r.w,
+                     (const double2*)vj,
+                     j > 0 ? (const double2*)(vj - r.vstride) : (const double2*)nullptr,
+                     (const double*)(r.d_alpha + j),
+                     (const double*)(j > 0 ? r.d_beta + j - 1 : r.d_beta), r.n, r.part, stop);
+  hipLaunchKernelGGL(k_fold_beta, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part,
+                     r.np, r.d_beta + j, r.thr, (int)j, r.st);
+  if (j > 3 && j % 2 == 0) {
+    const int m = (int)j + 1;
+    const int dst = (int)((j / 2) & 1);
+    const bool compare = j > 4;
+    launch_tri_coef(r, m, 0);
+    hipLaunchKernelGGL(k_combine, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.V, r.n,
+                       (int64_t)r.vstride, m, (const double2*)r.d_coef,
+                       compare ? (const double2*)r.res[dst ^ 1] : (const double2*)nullptr,
+                       r.res[dst], r.part, (const int*)r.st);
+    if (compare)
+      hipLaunchKernelGGL(k_fold_check, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part,
+                         r.np, (int)j, dst, r.st);
+  }
+  if (j + 1 < r.kcap)
+    hipLaunchKernelGGL(k_div, dim3(r.ew_blocks), dim3(256), 0, r.s, (const double2*)r.w,
+                       r.V + (size_t)(j + 1) * r.vstride, (const double*)(r.d_beta + j), r.n,
+                       stop);
+  RENO_CHECK_LAUNCH();
+  return RENO_OK;
+}
+
+// final combination / copy into out; m_max bounds MFINAL
+static int queue_final(KrylovRun& r, void* out, int m_max) {
+  launch_tri_coef(r, m_max, 1);
+  hipLaunchKernelGGL(k_final_out, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.V,
+                     r.n, (int64_t)r.vstride, (const double2*)r.d_coef,
+                     (const double2*)r.res[0], (const double2*)r.res[1], (double2*)out,
+                     (const int*)r.st);
+  RENO_CHECK_LAUNCH();
+  return RENO_OK;
+}
+
+static int check_args(const reno_hop_desc* op, const void* v0, const void* out) {
+  RENO_TRY(check_hop(op));
+  if (!v0 || !out) {
+    set_error("expm_krylov: null argument");
+    return RENO_EINVAL;
+  }
+  return RENO_OK;
 }
 
 }  // namespace reno
@@ -308,180 +578,88 @@ static hipEvent_t stage_event(size_t i) {
 using namespace reno;
 
 extern "C" size_t reno_krylov_workspace_bytes(const reno_hop_desc* op, int kcap) {
-  if (check_hop(op) != RENO_OK || kcap < 2) return 0;
+  if (check_hop(op) != RENO_OK || kcap < 2 || kcap > kMaxKrylov) return 0;
   return krylov_layout(op, kcap).total;
 }
 
 extern "C" int reno_expm_krylov(const reno_hop_desc* op, double dt_re, double dt_im,
                                 const void* v0, void* out, int* n_matvec, size_t* needed,
                                 void* ws, size_t ws_bytes, void* stream) {
-  RENO_TRY(check_hop(op));
-  if (!v0 || !out || !n_matvec) {
-    set_error("expm_krylov: null argument");
+  RENO_TRY(check_args(op, v0, out));
+  if (!n_matvec) {
+    set_error("expm_krylov: null n_matvec");
     return RENO_EINVAL;
   }
   hipStream_t s = (hipStream_t)stream;
-  const int64_t n = hop_vec_len(op);
-  int kcap = 2;
-  while (kcap < kMaxKrylov && krylov_layout(op, kcap + 1).total <= ws_bytes) ++kcap;
-  const KrylovLayout L = krylov_layout(op, kcap);
-  if (!ws || L.total > ws_bytes) {
+  const int kcap = kcap_for(op, ws_bytes);
+  if (!ws || krylov_layout(op, kcap).total > ws_bytes) {
     if (needed) *needed = krylov_layout(op, 16).total;
     set_error("expm_krylov: workspace %zu too small", ws_bytes);
     return RENO_ENOSPACE;
   }
-  double* pin = pinned_buf();
-  if (!pin) {
-    set_error("expm_krylov: pinned host buffer allocation failed");
-    return RENO_EHIP;
-  }
-  char* base = (char*)ws;
-  void* wl = base + L.off_wl;
-  void* t1 = base + L.off_t1;
-  void* t2 = base + L.off_t2;
-  void* split = base + L.off_split;
-  double2* V = (double2*)(base + L.off_V);
-  const size_t vstride = align_up((size_t)n * 16) / 16;
-  double2* w = (double2*)(base + L.off_w);
-  double2* res[2] = {(double2*)(base + L.off_r0), (double2*)(base + L.off_r1)};
-  double* part = (double*)(base + L.off_part);
-  double* scal = (double*)(base + L.off_scal);  // [0]=|v0| [1]=violations [4]=stop flag (int)
-  int* stop = (int*)(scal + 4);
-  double* d_alpha = scal + 8;
-  double* d_beta = scal + 8 + kcap;
-  double2* d_coef = (double2*)(base + L.off_coef);
-  const int np = n_partials(n);
-  const int ew_blocks = (int)((n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048);
-  const double thr = 100.0 * (double)n * 2.220446049250313e-16;  // krylov.py:72
-
-  if (op->nsite == 1) RENO_TRY(permute_w(op->prec, op->W, wl, op->Dl, op->d, op->Dr, s));
-
-  // |v0| and V[0] = v0 / |v0| (krylov.py:42-45); clears the stop flag
-  hipLaunchKernelGGL(k_norm2, dim3(np), dim3(kRedThreads), 0, s, (const double2*)v0, n, part);
-  hipLaunchKernelGGL(k_fold, dim3(1), dim3(kRedThreads), 0, s, (const double*)part, np, scal, 3,
-                     0.0, stop);
-  hipLaunchKernelGGL(k_div, dim3(ew_blocks), dim3(256), 0, s, (const double2*)v0, V,
-                     (const double*)scal, n, (const int*)nullptr);
-  RENO_CHECK_LAUNCH();
-  RENO_HIP_CHECK(hipMemcpyAsync(pin + kPinCoef - 8, scal, sizeof(double), hipMemcpyDeviceToHost, s));
-
-  // queue iteration j: matvec, alpha_j, and (j < n-1) the recurrence, beta_j
-  // and V[j+1]; then stage alpha_j, beta_j and the stop flag to the host.
-  auto queue_iter = [&](int64_t j) -> int {
-    double2* vj = V + (size_t)j * vstride;
-    RENO_TRY(hop_apply_core(op, wl, vj, w, t1, t2, s, stop, split, L.split_bytes));
-    hipLaunchKernelGGL(k_dot_re, dim3(np), dim3(kRedThreads), 0, s, (const double2*)w,
-                       (const double2*)vj, n, part, (const int*)stop);
-    hipLaunchKernelGGL(k_fold, dim3(1), dim3(kRedThreads), 0, s, (const double*)part, np,
-                       d_alpha + j, 0, 0.0, stop);
-    if (j < n - 1) {
-      hipLaunchKernelGGL(k_recur_norm2, dim3(np), dim3(kRedThreads), 0, s, w, (const double2*)vj,
-                         j > 0 ? (const double2*)(vj - vstride) : (const double2*)nullptr,
-                         (const double*)(d_alpha + j), (const double*)(j > 0 ? d_beta + j - 1 : d_beta),
-                         n, part, (const int*)stop);
-      hipLaunchKernelGGL(k_fold, dim3(1), dim3(kRedThreads), 0, s, (const double*)part, np,
-                         d_beta + j, 1, thr, stop);
-      if (j + 1 < kcap)
-        hipLaunchKernelGGL(k_div, dim3(ew_blocks), dim3(256), 0, s, (const double2*)w,
-                           V + (size_t)(j + 1) * vstride, (const double*)(d_beta + j), n,
-                           (const int*)stop);
-    }
-    RENO_CHECK_LAUNCH();
-    RENO_HIP_CHECK(hipMemcpyAsync(pin + 2 * j, d_alpha + j, sizeof(double), hipMemcpyDeviceToHost, s));
-    if (j < n - 1)
-      RENO_HIP_CHECK(hipMemcpyAsync(pin + 2 * j + 1, d_beta + j, sizeof(double), hipMemcpyDeviceToHost, s));
-    hipEvent_t e = stage_event((size_t)j);
-    if (!e) {
-      set_error("expm_krylov: event creation failed");
+  if (!t_stage.pin) {
+    if (hipHostMalloc((void**)&t_stage.pin, KS_INTS * sizeof(int), hipHostMallocDefault) !=
+        hipSuccess) {
+      t_stage.pin = nullptr;
+      set_error("expm_krylov: pinned host buffer allocation failed");
       return RENO_EHIP;
     }
-    RENO_HIP_CHECK(hipEventRecord(e, s));
-    return RENO_OK;
-  };
-
-  std::vector<double> alpha, beta;
-  std::vector<double2> coef;
-  int cur = 0;
-  bool have_res = false;
-  double nrmv = -1.0;
-
-  // the combination sum_i coef_i V[i] for m vectors; with check, also count
-  // allclose violations against res[cur] and raise the stop flag when none
-  auto queue_combine = [&](int m, double2* dst, bool check) -> int {
-    tridiag_expm_coef(m, alpha.data(), beta.data(), nrmv, dt_re, dt_im, coef);
-    memcpy(pin + kPinCoef, coef.data(), sizeof(double2) * m);
-    RENO_HIP_CHECK(hipMemcpyAsync(d_coef, pin + kPinCoef, sizeof(double2) * m,
-                                  hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_combine, dim3(np), dim3(kRedThreads), 0, s, (const double2*)V, n,
-                       (int64_t)vstride, m, (const double2*)d_coef,
-                       check ? (const double2*)res[cur] : (const double2*)nullptr, dst,
-                       check ? part : (double*)nullptr);
-    if (check)
-      hipLaunchKernelGGL(k_fold, dim3(1), dim3(kRedThreads), 0, s, (const double*)part, np,
-                         scal + 1, 2, 0.0, stop);
-    RENO_CHECK_LAUNCH();
-    return RENO_OK;
-  };
-
-  RENO_TRY(queue_iter(0));
-  for (int64_t j = 0; j < n; ++j) {
-    const bool last = (j == n - 1);
-    const bool check_pt = (j > 3 && j % 2 == 0 && !last);
-    // keep the device busy: queue the next iteration now unless this one
-    // needs a host decision first (allclose checks)
-    if (!check_pt && !last && j + 1 < kcap) RENO_TRY(queue_iter(j + 1));
-    RENO_HIP_CHECK(hipEventSynchronize(stage_event((size_t)j)));
-    if (nrmv < 0) {
-      nrmv = pin[kPinCoef - 8];
-      if (!(nrmv > 0.0)) {
-        set_error("expm_krylov: zero starting vector (krylov.py:44 assert)");
-        return RENO_EINVAL;
-      }
-    }
-    alpha.push_back(pin[2 * j]);
-    if (last) {
-      // Krylov space equals the full space (krylov.py:59-61)
-      RENO_TRY(queue_combine((int)j + 1, (double2*)out, false));
-      RENO_HIP_CHECK(hipStreamSynchronize(s));  // pinned coefficients in flight
-      *n_matvec = (int)j + 1;
-      return RENO_OK;
-    }
-    beta.push_back(pin[2 * j + 1]);
-    if (beta[j] < thr) {
-      // Lanczos breakdown (krylov.py:72-74); the queued iteration j+1 was a no-op
-      RENO_TRY(queue_combine((int)j + 1, (double2*)out, false));
-      RENO_HIP_CHECK(hipStreamSynchronize(s));
-      *n_matvec = (int)j + 1;
-      return RENO_OK;
-    }
-    if (check_pt) {
-      const int m = (int)j + 1;
-      RENO_TRY(queue_combine(m, res[cur ^ 1], have_res));
-      hipEvent_t ec = stage_event((size_t)kMaxKrylov);
-      if (have_res) {
-        RENO_HIP_CHECK(hipMemcpyAsync(pin + kPinCoef - 4, scal + 1, sizeof(double),
-                                      hipMemcpyDeviceToHost, s));
-      }
-      RENO_HIP_CHECK(hipEventRecord(ec, s));
-      // the next iteration goes behind the check; it is cancelled on device
-      // if the check converged
-      if (j + 1 < kcap) RENO_TRY(queue_iter(j + 1));
-      RENO_HIP_CHECK(hipEventSynchronize(ec));
-      if (have_res && pin[kPinCoef - 4] == 0.0) {
-        RENO_HIP_CHECK(hipMemcpyAsync(out, res[cur ^ 1], (size_t)n * 16, hipMemcpyDeviceToDevice, s));
-        *n_matvec = (int)j + 1;
-        return RENO_OK;
-      }
-      cur ^= 1;
-      have_res = true;
-    }
-    if (j + 1 >= kcap) {
-      if (needed) *needed = krylov_layout(op, 2 * kcap).total;
-      RENO_HIP_CHECK(hipStreamSynchronize(s));
-      set_error("expm_krylov: %d Krylov vectors do not fit the workspace", kcap);
-      return RENO_ENOSPACE;
-    }
   }
-  set_error("expm_krylov: unreachable");
-  return RENO_EINVAL;
+  if (!t_stage.ev) RENO_HIP_CHECK(hipEventCreateWithFlags(&t_stage.ev, hipEventDisableTiming));
+  KrylovRun r;
+  bind_run(r, op, s, ws, kcap, dt_re, dt_im);
+  RENO_TRY(queue_start(r, v0));
+  // chunks of iterations; the host reads the device state once per chunk
+  int64_t j = 0;
+  int64_t chunk = 12;
+  const int64_t jmax = std::min<int64_t>(r.n, kcap);
+  while (true) {
+    const int64_t jend = std::min<int64_t>(j + chunk, jmax);
+    for (; j < jend; ++j) RENO_TRY(queue_iter(r, j));
+    RENO_HIP_CHECK(hipMemcpyAsync(t_stage.pin, r.st, KS_INTS * sizeof(int),
+                                  hipMemcpyDeviceToHost, s));
+    RENO_HIP_CHECK(hipEventRecord(t_stage.ev, s));
+    RENO_HIP_CHECK(hipEventSynchronize(t_stage.ev));
+    const int status = t_stage.pin[KS_STATUS];
+    if (status < 0) {
+      set_error("expm_krylov: zero starting vector (krylov.py:44 assert)");
+      return RENO_EINVAL;
+    }
+    if (status > 0) {
+      *n_matvec = t_stage.pin[KS_NMATVEC];
+      return queue_final(r, out, (int)jend);
+    }
+    if (j >= jmax) {
+      if (needed) *needed = krylov_layout(op, std::min(2 * kcap, kMaxKrylov)).total;
+      set_error("expm_krylov: %d Krylov vectors do not fit the workspace", kcap);
+      return kcap >= kMaxKrylov ? RENO_EINVAL : RENO_ENOSPACE;
+    }
+    chunk = 8;
+  }
+}
+
+extern "C" int reno_expm_krylov_async(const reno_hop_desc* op, double dt_re, double dt_im,
+                                      const void* v0, void* out, int max_matvec, int* info,
+                                      void* ws, size_t ws_bytes, void* stream) {
+  RENO_TRY(check_args(op, v0, out));
+  if (!info || max_matvec < 1) {
+    set_error("expm_krylov_async: null info or max_matvec < 1");
+    return RENO_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int kcap = kcap_for(op, ws_bytes);
+  const int64_t jmax = std::min<int64_t>(hop_vec_len(op), max_matvec);
+  if (!ws || krylov_layout(op, kcap).total > ws_bytes || jmax > kcap) {
+    set_error("expm_krylov_async: workspace %zu holds %d Krylov vectors, %lld requested",
+              ws_bytes, kcap, (long long)jmax);
+    return RENO_ENOSPACE;
+  }
+  KrylovRun r;
+  bind_run(r, op, s, ws, kcap, dt_re, dt_im);
+  RENO_TRY(queue_start(r, v0));
+  for (int64_t j = 0; j < jmax; ++j) RENO_TRY(queue_iter(r, j));
+  RENO_TRY(queue_final(r, out, (int)jmax));
+  hipLaunchKernelGGL(k_info, dim3(1), dim3(1), 0, s, (const int*)r.st, (int)jmax, info);
+  RENO_CHECK_LAUNCH();
+  return RENO_OK;
 }

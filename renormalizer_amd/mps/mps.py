@@ -292,32 +292,73 @@ class Mps(MatrixProduct):
         if self.evolve_config.ivp_solver != "krylov":
             raise NotImplementedError("only the Krylov propagator is on the device path")
         prec = _PREC[self.evolve_config.precision]
+        n = len(self)
+        cache = self._env_cache
+        reuse = cache is not None and cache[0] is mpo and cache[1] == prec and not self.to_right \
+            and self.qnidx == n - 1
+        # the sweep rewrites the cached environments in place: take them away
+        # from self so that a second evolution of self rebuilds its own
+        self._env_cache = None
+        budgets = None
+        if reuse and self.evolve_config.krylov_async and cache[3] is not None \
+                and cache[3][0] == complex(evolve_dt):
+            budgets = cache[3][1]
+        mps, environ, counts = self._tdvp_ps_sweep(mpo, evolve_dt, prec, cache[2] if reuse else None,
+                                                   budgets)
+        if counts is None:
+            # an iteration budget was too small: redo the step with host waits
+            mps, environ, counts = self._tdvp_ps_sweep(mpo, evolve_dt, prec, None, None)
+        local_steps = [c for _, c in counts]
+        mps.evolve_config.stat = stats.describe(local_steps)
+        mps.evolve_config.local_steps = local_steps
+        mps._env_cache = (mpo, prec, environ, (complex(evolve_dt), dict(counts)))
+        return mps
+
+    # number of time steps redone with host waits because an asynchronous
+    # Krylov budget proved too small (diagnostics)
+    krylov_redo_count = 0
+
+    def _tdvp_ps_sweep(self, mpo, evolve_dt, prec, environ, budgets):
+        """The sweep itself.  With ``budgets`` (matvec counts of the previous
+        step keyed by position) every Krylov call is queued asynchronously and
+        the counts are read once at the end; returns counts None when a
+        budget ran out."""
         mps = self.to_complex()
         if not mps.on_device:
             mps.to_device()
         dev = mps._mp[0].device
         wt = mpo.device_tensors(prec, dev)
         n = len(mps)
-        cache = self._env_cache
-        if cache is not None and cache[0] is mpo and cache[1] == prec and not mps.to_right \
-                and mps.qnidx == n - 1:
-            # the sweep below rewrites these environments in place: take them
-            # away from self so a second evolution of self rebuilds its own
-            environ = cache[2]
-            self._env_cache = None
-        else:
+        if environ is None:
             # only the left environments are read before being rebuilt
             environ = Environ(mps._mp, wt, "L" if not mps.to_right else "R", prec=prec)
-        local_steps = []
-        for _ in range(2):
+        use_async = budgets is not None
+        if use_async:
+            info = torch.zeros((4 * n + 4, 2), dtype=torch.int32, device=dev)
+        keys = []
+        counts = []
+
+        def krylov(key, l_array, w, r_array, v, dt):
+            if not use_async:
+                out, j = native.expm_krylov(l_array, w, r_array, v, dt, prec)
+                counts.append((key, j))
+                return out
+            # the previous step's count at the same place, plus a quarter and
+            # four more convergence checks
+            prev = budgets.get(key, 0)
+            budget = max(prev + prev // 4 + 8, 12)
+            out = native.expm_krylov_async(l_array, w, r_array, v, dt, prec, budget, info[len(keys)])
+            keys.append(key)
+            return out
+
+        for half in range(2):
             for imps in mps.iter_idx_list(full=True):
                 system = "L" if mps.to_right else "R"
                 l_array = environ.read("L", imps - 1)
                 r_array = environ.read("R", imps + 1)
                 site = mps._mp[imps]
                 shape = list(site.shape)
-                mps_t, j = native.expm_krylov(l_array, wt[imps], r_array, site, -1j * evolve_dt / 2, prec)
-                local_steps.append(j)
+                mps_t = krylov((half, imps, 1), l_array, wt[imps], r_array, site, -1j * evolve_dt / 2)
                 qnbigl, qnbigr, _ = mps._get_big_qn([imps])
                 u, qnlset, vt, qnrset = svd_qn_device(mps_t, qnbigl, qnbigr, mps.qntot, system)
                 if not mps.to_right and imps != 0:
@@ -326,8 +367,7 @@ class Mps(MatrixProduct):
                     mps.qn[imps] = np.array(qnrset, dtype=int).reshape(kt, -1)
                     mps.qnidx = imps - 1
                     r_array = environ.GetLR("R", imps, mps._mp, wt, itensor=r_array, method="System")
-                    u_t, j = native.expm_krylov(l_array, None, r_array, u, 1j * evolve_dt / 2, prec)
-                    local_steps.append(j)
+                    u_t = krylov((half, imps, 0), l_array, None, r_array, u, 1j * evolve_dt / 2)
                     prev = mps._mp[imps - 1]
                     mps._mp[imps - 1] = native.zgemm(prev.reshape(-1, prev.shape[-1]), u_t).reshape(
                         list(prev.shape[:-1]) + [kt])
@@ -337,18 +377,27 @@ class Mps(MatrixProduct):
                     mps.qn[imps + 1] = np.array(qnlset, dtype=int).reshape(kt, -1)
                     mps.qnidx = imps + 1
                     l_array = environ.GetLR("L", imps, mps._mp, wt, itensor=l_array, method="System")
-                    vt_t, j = native.expm_krylov(l_array, None, r_array, vt, 1j * evolve_dt / 2, prec)
-                    local_steps.append(j)
+                    vt_t = krylov((half, imps, 0), l_array, None, r_array, vt, 1j * evolve_dt / 2)
                     nxt = mps._mp[imps + 1]
                     mps._mp[imps + 1] = native.zgemm(vt_t, nxt.reshape(nxt.shape[0], -1)).reshape(
                         [kt] + list(nxt.shape[1:]))
                 else:
                     mps._mp[imps] = mps_t
             mps._switch_direction()
-        mps.evolve_config.stat = stats.describe(local_steps)
-        mps.evolve_config.local_steps = local_steps
-        mps._env_cache = (mpo, prec, environ)
-        return mps
+        if use_async:
+            got = info[:len(keys)].cpu().numpy()
+            # a call that ran out of budget left its output unwritten, so
+            # everything after it is void: redo (checked before the zero-vector
+            # flag, which such garbage can raise downstream)
+            if (got[:, 1] == 0).any():
+                Mps.krylov_redo_count += 1
+                logger.info("Krylov budget exhausted at %s; redoing the step with host waits",
+                            [k for k, g in zip(keys, got) if g[1] == 0][:4])
+                return mps, environ, None
+            if (got[:, 1] < 0).any():
+                raise AssertionError("zero vector handed to expm_krylov (krylov.py:44)")
+            counts = [(k, int(c)) for k, c in zip(keys, got[:, 0])]
+        return mps, environ, counts
 
     # ---- observables --------------------------------------------------------
     def expectation(self, mpo: Union[Mpo, Op, OpSum], self_conj: "Mps" = None):

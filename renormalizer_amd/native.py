@@ -32,7 +32,7 @@ EXPORTS = (
     "reno_hop_workspace_bytes", "reno_krylov_workspace_bytes",
     "reno_env_workspace_bytes", "reno_qr_workspace_bytes",
     "reno_hop_apply", "reno_env_update", "reno_env_close", "reno_expm_krylov",
-    "reno_block_qr", "reno_zgemm", "reno_probe_mfma16", "reno_prof_enable",
+    "reno_expm_krylov_async", "reno_block_qr", "reno_zgemm", "reno_probe_mfma16", "reno_prof_enable",
     "reno_prof_collect",
 )
 
@@ -97,6 +97,9 @@ def load_library(path: str = LIB_PATH):
         lib.reno_expm_krylov.argtypes = [ctypes.POINTER(HopDesc), c_double, c_double, vp, vp,
                                          ctypes.POINTER(c_int), ctypes.POINTER(c_size),
                                          vp, c_size, vp]
+        lib.reno_expm_krylov_async.restype = c_int
+        lib.reno_expm_krylov_async.argtypes = [ctypes.POINTER(HopDesc), c_double, c_double, vp, vp,
+                                               c_int, vp, vp, c_size, vp]
         lib.reno_block_qr.restype = c_int
         lib.reno_block_qr.argtypes = [c_int, c_int, c_int, vp, c_int, ctypes.POINTER(QnBlock),
                                       vp, vp, c_int, vp, c_size, vp]
@@ -257,11 +260,7 @@ def env_update(env, ms, W, domain, ms_conj=None, prec=PREC_FP64):
 _KRYLOV_KCAP = 20
 
 
-def expm_krylov(L, W, R, v, dt, prec=PREC_FP64):
-    """expm(dt * H_eff) v by Lanczos (reference lib/krylov/krylov.py:27-84).
-
-    Returns (vector, number of matvecs) like the reference.
-    """
+def _krylov_desc(L, W, R, v, prec):
     _require_device(L, R, v, W)
     _check_env_dtype(prec, L, R, W)
     if v.dtype != torch.complex128 or not v.is_contiguous():
@@ -270,12 +269,20 @@ def expm_krylov(L, W, R, v, dt, prec=PREC_FP64):
         Ml, Mr = v.shape
         D = L.shape[1]
         assert L.shape == (Ml, D, Ml) and R.shape == (Mr, D, Mr), "hop0 shape mismatch"
-        desc = _hop_desc(0, prec, L, None, R, Ml, 1, Mr, D, D)
-    else:
-        Ml, d, Mr = v.shape
-        Dl, Dr = W.shape[0], W.shape[3]
-        assert L.shape == (Ml, Dl, Ml) and R.shape == (Mr, Dr, Mr), "hop1 shape mismatch"
-        desc = _hop_desc(1, prec, L, W, R, Ml, d, Mr, Dl, Dr)
+        return _hop_desc(0, prec, L, None, R, Ml, 1, Mr, D, D)
+    Ml, d, Mr = v.shape
+    Dl, Dr = W.shape[0], W.shape[3]
+    assert L.shape == (Ml, Dl, Ml) and R.shape == (Mr, Dr, Mr), "hop1 shape mismatch"
+    return _hop_desc(1, prec, L, W, R, Ml, d, Mr, Dl, Dr)
+
+
+def expm_krylov(L, W, R, v, dt, prec=PREC_FP64):
+    """expm(dt * H_eff) v by Lanczos (reference lib/krylov/krylov.py:27-84).
+
+    Returns (vector, number of matvecs) like the reference.  Waits for the
+    device once per chunk of iterations (normally once per call).
+    """
+    desc = _krylov_desc(L, W, R, v, prec)
     dt = complex(dt)
     l = lib()
     kcap = _KRYLOV_KCAP
@@ -293,6 +300,34 @@ def expm_krylov(L, W, R, v, dt, prec=PREC_FP64):
             continue
         check(st, "reno_expm_krylov")
         return out, nmv.value
+
+
+# largest Krylov space of one asynchronous call
+KRYLOV_ASYNC_MAX = 1024
+
+
+def expm_krylov_async(L, W, R, v, dt, prec, max_matvec, info):
+    """Queue expm(dt * H_eff) v without waiting for the device.
+
+    At most ``max_matvec`` Lanczos iterations are queued.  ``info`` is a
+    device int32 tensor of two elements that receives (matvecs used, flag):
+    flag 1 when the reference's stopping rule ended the iteration, 0 when
+    the budget ran out first (the returned vector is then not the result and
+    the caller must redo the call with ``expm_krylov``), -1 for a zero v.
+    """
+    desc = _krylov_desc(L, W, R, v, prec)
+    if info.dtype != torch.int32 or info.numel() < 2 or not info.is_cuda:
+        raise ValueError("info must be a device int32 tensor of two elements")
+    max_matvec = int(max(2, min(max_matvec, KRYLOV_ASYNC_MAX)))
+    dt = complex(dt)
+    l = lib()
+    out = torch.empty_like(v)
+    nbytes = l.reno_krylov_workspace_bytes(ctypes.byref(desc), max_matvec)
+    ws = workspace.get(nbytes, v.device)
+    check(l.reno_expm_krylov_async(ctypes.byref(desc), dt.real, dt.imag, _ptr(v), _ptr(out),
+                                   max_matvec, _ptr(info), _ptr(ws), ws.numel(), _stream()),
+          "reno_expm_krylov_async")
+    return out
 
 
 def block_qr(system, coef, blocks, v_transposed=False):

@@ -159,7 +159,8 @@ class EvolveConfig:
     def __init__(self, method: Union[EvolveMethod, str] = EvolveMethod.prop_and_compress,
                  adaptive=False, guess_dt=1e-1, adaptive_rtol=5e-4, taylor_order: int = None,
                  rk_solver="C_RK4", reg_epsilon=1e-10, ivp_rtol=1e-5, ivp_atol=1e-8,
-                 ivp_solver="krylov", force_ovlp=True, precision: str = "fp64"):
+                 ivp_solver="krylov", force_ovlp=True, precision: str = "fp64",
+                 krylov_async: bool = True):
         if isinstance(method, str):
             method = getattr(EvolveMethod, method)
         self.method = method
@@ -181,6 +182,10 @@ class EvolveConfig:
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {PRECISIONS}")
         self.precision = precision
+        # device sweep: queue the Lanczos iterations of a whole time step
+        # without waiting for the device (iteration budgets from the previous
+        # step; the step is redone with waits when a budget proves too small)
+        self.krylov_async = bool(krylov_async)
         self.stat = None
 
     @property

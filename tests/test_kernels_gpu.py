@@ -167,6 +167,58 @@ def test_expm_krylov_matches_oracle(dev, shape):
     assert _rel(out.cpu().numpy().ravel(), ref) < 1e-11
 
 
+@pytest.mark.parametrize("shape,dt", [
+    ((1, 2, 2, 1, 3), -0.05j / 2),     # full-space exit (n = 4)
+    ((16, 8, 16, 3, 3), -0.05j / 2),
+    ((16, 8, 16, 3, 3), -3.0j),        # long Krylov space: several host chunks, Taylor substeps
+    ((16, 8, 16, 3, 3), -0.4),         # imaginary time (real dt)
+    ((64, 10, 64, 3, 3), -0.05j / 2),
+])
+def test_expm_krylov_async_matches_sync_and_oracle(dev, shape, dt):
+    """The asynchronous entry point runs the same kernels as the synchronous
+    one (bit-identical vector, same count); both match the oracle."""
+    print(f"[krylov async {shape} dt={dt}]", flush=True)
+    Ml, d, Mr, Dl, Dr = shape
+    rng = np.random.default_rng(23 + sum(shape))
+    L, W, R = _herm_env_inputs(rng, *shape)
+    v = _rand(rng, (Ml, d, Mr))
+    args = (_t(L, dev), _t(W, dev), _t(R, dev), _t(v, dev))
+    out, nmv = native.expm_krylov(*args, dt)
+    ref, nref = orc.expm_krylov(lambda y: orc.hop1(L, W, R, y.reshape(v.shape)).ravel(), dt, v.ravel())
+    assert nmv == nref
+    assert _rel(out.cpu().numpy().ravel(), ref) < 1e-11
+    info = torch.zeros(2, dtype=torch.int32, device=dev)
+    out_a = native.expm_krylov_async(*args, dt, native.PREC_FP64, nmv + 5, info)
+    assert info.cpu().tolist() == [nmv, 1]
+    assert torch.equal(out_a, out)
+    if nmv > 8:
+        # budget too small: reported, not silently wrong
+        info.zero_()
+        native.expm_krylov_async(*args, dt, native.PREC_FP64, 6, info)
+        assert info.cpu().tolist() == [6, 0]
+
+
+def test_expm_krylov_breakdown(dev):
+    """H_eff = identity: beta_0 = 0, breakdown after one matvec, exp(dt) v."""
+    print("[krylov breakdown]", flush=True)
+    Ml, d, Mr = 6, 3, 5
+    L = np.eye(Ml).reshape(Ml, 1, Ml).astype(complex)
+    R = np.eye(Mr).reshape(Mr, 1, Mr).astype(complex)
+    W = np.eye(d).reshape(1, d, d, 1)
+    rng = np.random.default_rng(2)
+    v = _rand(rng, (Ml, d, Mr))
+    dt = -0.3j
+    args = (_t(L, dev), _t(W, dev), _t(R, dev), _t(v, dev))
+    out, nmv = native.expm_krylov(*args, dt)
+    ref, nref = orc.expm_krylov(lambda y: orc.hop1(L, W, R, y.reshape(v.shape)).ravel(), dt, v.ravel())
+    assert nmv == nref == 1
+    np.testing.assert_allclose(out.cpu().numpy(), np.exp(dt) * v, rtol=1e-13, atol=1e-14)
+    info = torch.zeros(2, dtype=torch.int32, device=dev)
+    out_a = native.expm_krylov_async(*args, dt, native.PREC_FP64, 12, info)
+    assert info.cpu().tolist() == [1, 1]
+    assert torch.equal(out_a, out)
+
+
 def test_expm_krylov_bond(dev):
     print("[krylov bond]", flush=True)
     rng = np.random.default_rng(5)

@@ -216,3 +216,45 @@ def test_tdvp_reference_holstein_target():
         got.append(helpers.e_occupations_oracle(st, model))
         ref.append([dense.site_rdm(exact[k], dims, 2 * i)[1, 1].real for i in range(3)])
     assert np.abs(np.array(got) - np.array(ref)).mean() < 1e-4
+
+
+def _random_sites(rng, dims, bonds):
+    return [rng.standard_normal((bonds[i], d, bonds[i + 1])) + 1j * rng.standard_normal((bonds[i], d, bonds[i + 1]))
+            for i, d in enumerate(dims)]
+
+
+def test_dot_norm_expectation_rdm_bruteforce():
+    """dot / mp_norm / normalize / expectation / calc_1site_rdm against the
+    dense state vector of a random 4-site MPS and a random dense MPO."""
+    rng = np.random.default_rng(11)
+    dims, bonds = [2, 3, 2, 3], [1, 2, 4, 3, 1]
+    a = _random_sites(rng, dims, bonds)
+    b = _random_sites(rng, dims, bonds)
+    def state(sites, qnidx):
+        return orc.State([s.copy() for s in sites], [np.zeros((x, 1), int) for x in bonds], qnidx,
+                         np.zeros(1, int), False, [np.zeros(d, int) for d in dims])
+
+    va = _to_dense(state(a, 0))
+    vb = _to_dense(state(b, 0))
+    # <a|b> with a passed conjugated (reference dot convention)
+    np.testing.assert_allclose(orc.dot([s.conj() for s in a], b), np.vdot(va, vb), rtol=1e-12)
+    st = state(a, 2)
+    np.testing.assert_allclose(orc.mp_norm(st), np.linalg.norm(va), rtol=1e-12)
+    orc.normalize_mps_only(st)
+    np.testing.assert_allclose(np.linalg.norm(_to_dense(st)), 1.0, rtol=1e-12)
+    # Hermitian MPO with bond dims 2: dense operator by brute force
+    mbond = [1, 2, 2, 2, 1]
+    w = []
+    for i, d in enumerate(dims):
+        x = rng.standard_normal((mbond[i], d, d, mbond[i + 1]))
+        w.append(x + x.transpose(0, 2, 1, 3))
+    op = np.ones((1, 1, 1))
+    for x in w:
+        op = np.einsum("ija,aklb->ikjlb", op, x).reshape(op.shape[0] * x.shape[1], op.shape[1] * x.shape[2], -1)
+    op = op[:, :, 0]
+    vn = _to_dense(st)
+    np.testing.assert_allclose(orc.expectation(st, w), np.vdot(vn, op @ vn).real, rtol=1e-11)
+    for site in range(4):
+        rdm = orc.calc_1site_rdm(st, site)
+        ref = dense.site_rdm(vn, dims, site)
+        np.testing.assert_allclose(rdm, ref, atol=1e-12)

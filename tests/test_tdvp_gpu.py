@@ -166,3 +166,34 @@ def test_spin_boson_c1_shape_parity():
     od, oo, sd, so, *_ = _run_pair(mps, mpo, 0.05, 10, _sz_dev, _sz_orc)
     np.testing.assert_allclose(od, oo, atol=1e-8, rtol=0)
     _assert_steps_match(sd, so)
+
+
+def test_async_budget_redo_matches_sync():
+    """Asynchronous Krylov calls with budgets that are too small: the step is
+    redone with host waits and the state equals the synchronous step."""
+    print("[tdvp async redo sbm 6x4 M=8]", flush=True)
+    from renormalizer_amd import Mps
+    model = helpers.sbm_model(6, 4, alpha=0.5, omega_c=5.0)
+    mps, mpo = helpers.sbm_initial(model, 8)
+    mps.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps)
+    mps.to_device()
+    first = mps.evolve(mpo, 0.5)  # synchronous (no budgets yet), big dt: long Krylov spaces
+    assert max(first.evolve_config.local_steps) > 12
+    sync = first.copy()
+    sync.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps, krylov_async=False)
+    sync = sync.evolve(mpo, 0.5)
+    # shrink the recorded budgets so every asynchronous call runs out
+    cache = first._env_cache
+    first._env_cache = cache[:3] + ((cache[3][0], {k: 1 for k in cache[3][1]}),)
+    before = Mps.krylov_redo_count
+    redone = first.evolve(mpo, 0.5)
+    assert Mps.krylov_redo_count == before + 1
+    assert redone.evolve_config.local_steps == sync.evolve_config.local_steps
+    for a, b in zip(redone.numpy_sites(), sync.numpy_sites()):
+        np.testing.assert_allclose(a, b, atol=1e-10)
+    # with the budgets of the redone step the next step runs asynchronously
+    # (or is redone again if a count grew past its margin): same result
+    nxt = redone.evolve(mpo, 0.5)
+    ref = sync.evolve(mpo, 0.5)
+    assert nxt.evolve_config.local_steps == ref.evolve_config.local_steps
+    np.testing.assert_allclose(_sz_dev(nxt), _sz_dev(ref), atol=1e-10)
