@@ -4,22 +4,31 @@
 // H_j = I - tau_j v_j v_j^H with v_j[j] = 1, beta_j = -sign(Re alpha) |(alpha,x)|
 // real, so R has a real diagonal; Q = H_0 H_1 ... H_{k-1}.
 //
-// Panels of NB = 8 columns.  One 512-thread workgroup factors a panel with the
-// panel held in registers (rows j0 + tid + 512 i), so each column step is a
-// fused norm reduction, reflector, dot-product reduction and update with no
-// trips to memory; it also forms the compact-WY factor T (zlarft, forward
-// columnwise).  The trailing matrix is updated as A += V (-T^H) (V^H A) by
-// one fused kernel per panel (each workgroup owns 4 columns: reduce V^H A
-// over the rows, multiply by the 8x8 factor, apply the rank-8 update).  Q is
-// formed by backward accumulation Q <- (I - V T V^H) Q with the same kernel.
+// R phase, per panel of NB = 8 columns: (a) a small launch applies the
+// previous panel's block reflector to the new panel's columns (one workgroup
+// per column); (b) k_qr_step: workgroup 0 factors the panel with its columns
+// held in registers (rows r0 + tid + 256 i), one reduction per column (norm of
+// x, the dot products with the later panel columns; the pivot row goes
+// through LDS), and forms the panel's compact-WY factor T (zlarft, forward
+// columnwise); the other workgroups meanwhile apply the previous panel's
+// reflector to one trailing column each.  Reductions use DPP lane moves
+// (VALU) rather than LDS shuffles; barriers order LDS only.
+//
+// Q phase (zungqr): with the explicit V (m x k, unit lower trapezoidal) and
+// the full k x k factor T of H_0...H_{k-1} = I - V T V^H,
+//   Q = (I - V T V^H) [I; 0] = E + V (N V1^H),  N = -T, V1 = V[0:k, :],
+// i.e. two matrix-core GEMMs.  N is assembled from the per-panel factors by
+// recursive doubling, N12 = N11 (V1^H V2) N22 for adjacent blocks, with the
+// Gram matrix V^H V from one more GEMM.
+#include <algorithm>
+
 #include "householder.h"
 
 namespace reno {
 
 constexpr int kQrNB = 8;
-constexpr int kPanelThreads = 512;
-constexpr int kApplyThreads = 256;
-constexpr int kApplyCols = 4;
+constexpr int kPanelThreads = 256;  // one wave per SIMD: up to 512 registers per lane
+constexpr int kMaxRpt = 10;         // rows per thread: m <= 2560
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -28,49 +37,209 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a) * b
   return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
-// Sum `nv` doubles across the workgroup; result broadcast in out[0..nv).
-// sh must hold (blockDim/64) * nv doubles.
-template <int NT, int NV>
-__device__ __forceinline__ void block_reduce(double (&vals)[NV], int nv, double* sh, double* out) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  constexpr int NW = NT / 64;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    if (i >= nv) break;
-    double v = vals[i];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) sh[wave * nv + i] = v;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < nv; i += NT) {
-    double s = 0.0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) s += sh[w * nv + i];
-    out[i] = s;
-  }
-  __syncthreads();
+// Workgroup barrier ordering LDS only: every exchange between the threads
+// of these kernels goes through LDS, so a barrier need not wait for the
+// global stores of the factored columns (a plain __syncthreads would).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Factor columns [j0, j0+jb) of X (m x ld, column-major), rows j0..m-1.
-// Writes R / v in place, tau[j0..j0+jb), the explicit V into Vall
-// (column j0+c, all rows, zeros above the unit diagonal), and into Tb the
-// row-major 8x8 factors -T^H (Tb[0..63], trailing update) and -T
-// (Tb[64..127], forming Q).
-template <int RPT>
-__global__ __launch_bounds__(kPanelThreads) void k_panel_qr(double2* __restrict__ X, int m, int ld,
-                                                            int j0, int jb,
-                                                            double2* __restrict__ tau,
-                                                            double2* __restrict__ Vall,
-                                                            double2* __restrict__ Tb) {
-  __shared__ double sh[(kPanelThreads / 64) * 64];
-  __shared__ double red[64];
-  __shared__ double2 sT[kQrNB][kQrNB];
-  __shared__ double2 sS[kQrNB][kQrNB];
-  __shared__ double2 s_alpha;
+// Sum of a double over the 64 lanes of a wavefront with DPP lane moves (VALU
+// only; no LDS crossbar traffic): quad xor 1 and 2, half-row mirror, row
+// mirror, then row_bcast:15 / row_bcast:31 fold the four rows into lane 63,
+// whose value is broadcast.  Fixed order: deterministic.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_f64<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141, 0xF>(v);  // row_half_mirror
+  v += dpp_f64<0x140, 0xF>(v);  // row_mirror
+  v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum NV doubles across the workgroup; results in out[0..NV).
+// sh holds (kPanelThreads/64) * NV doubles.  Fixed order: deterministic.
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&vals)[NV], double* sh, double* out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int NW = kPanelThreads / 64;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const double v = wave_sum(vals[i]);
+    if (lane == 0) sh[wave * NV + i] = v;
+  }
+  lds_barrier();
+  for (int i = threadIdx.x; i < NV; i += kPanelThreads) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += sh[w * NV + i];
+    out[i] = s;
+  }
+  lds_barrier();
+}
+
+// g = F (sum over the workgroup of w) for the 8x8 factor F (LDS): the
+// reduction, then 8 threads form g in LDS (keeps F out of the registers of
+// every thread).
+__device__ __forceinline__ void reduce_times_F(double (&w)[2 * kQrNB], const double2* F, double* sh,
+                                               double* red, double2* sG) {
+  block_reduce<2 * kQrNB>(w, sh, red);
+  if (threadIdx.x < kQrNB) {
+    const int q = threadIdx.x;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int p = 0; p < kQrNB; ++p)
+      acc = cadd(acc, cmul(F[q * kQrNB + p], make_double2(red[2 * p], red[2 * p + 1])));
+    sG[q] = acc;
+  }
+  lds_barrier();
+}
+
+// Column c (rows jp..m-1) <- (I + V F V^H) column, V = Vall[:, jp:jp+jbp],
+// F the row-major 8x8 factor (-T^H of panel jp) in LDS.  Rolled row loops:
+// V is streamed twice from L2, the registers stay small.
+__device__ void apply_prev_column(double2* __restrict__ X, int m, int ld, int c, int jp, int jbp,
+                                  const double2* __restrict__ Vall, const double2* F,
+                                  double* sh, double* red, double2* sG) {
   const int tid = threadIdx.x;
-  double2 a[RPT][kQrNB];
+  double w[2 * kQrNB];
+#pragma unroll
+  for (int q = 0; q < 2 * kQrNB; ++q) w[q] = 0.0;
+#pragma unroll 2
+  for (int r = jp + tid; r < m; r += kPanelThreads) {
+    const double2 x = X[r + (int64_t)c * ld];
+#pragma unroll
+    for (int q = 0; q < kQrNB; ++q) {
+      const double2 v = q < jbp ? Vall[r + (int64_t)(jp + q) * m] : make_double2(0.0, 0.0);
+      const double2 p = cmulc(v, x);
+      w[2 * q] += p.x;
+      w[2 * q + 1] += p.y;
+    }
+  }
+  reduce_times_F(w, F, sh, red, sG);
+  double2 g[kQrNB];
+#pragma unroll
+  for (int q = 0; q < kQrNB; ++q) g[q] = sG[q];
+#pragma unroll 2
+  for (int r = jp + tid; r < m; r += kPanelThreads) {
+    double2 y = X[r + (int64_t)c * ld];
+#pragma unroll
+    for (int q = 0; q < kQrNB; ++q) {
+      const double2 v = q < jbp ? Vall[r + (int64_t)(jp + q) * m] : make_double2(0.0, 0.0);
+      y = cadd(y, cmul(v, g[q]));
+    }
+    X[r + (int64_t)c * ld] = y;
+  }
+}
+
+// The panel columns, one runtime loop iteration per column jj (the code is
+// kept small: a fully unrolled panel overflows the instruction cache).  The
+// registers hold the not-yet-factored columns rotated so that column jj is
+// always slot 0 (slot c = column jj + c); after each step the slots shift
+// left by one, so every register index is a compile-time constant.  One
+// reduction per column: |x|^2 and g_c = sum_{r>j} conj(a_rj) a_rc; the pivot
+// row (alpha, a_jc) goes through LDS (double-buffered by column parity).
+// Each column is written to X / Vall as soon as it is final.
+template <int RPT>
+__device__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, int m, int ld, int j0, int jb,
+                             double2* __restrict__ X, double2* __restrict__ Vall, double* sh,
+                             double* red, double2 (*piv)[kQrNB], double2* sTau) {
+  const int tid = threadIdx.x;
+#pragma unroll 1
+  for (int jj = 0; jj < jb; ++jj) {
+    const int j = j0 + jj;
+    const int nc = jb - jj;  // live slots
+    double2* pv = piv[jj & 1];
+    double vals[1 + 2 * kQrNB];
+#pragma unroll
+    for (int t = 0; t < 1 + 2 * kQrNB; ++t) vals[t] = 0.0;
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = r0 + tid + kPanelThreads * i;
+      if (r > j && r < m) {
+        const double2 x = a[i][0];
+        vals[0] += x.x * x.x + x.y * x.y;
+#pragma unroll
+        for (int c = 1; c < kQrNB; ++c) {
+          const double2 p = cmulc(x, a[i][c]);
+          vals[1 + 2 * c] += p.x;
+          vals[2 + 2 * c] += p.y;
+        }
+      } else if (r == j) {
+#pragma unroll
+        for (int c = 0; c < kQrNB; ++c) pv[c] = a[i][c];
+      }
+    }
+    block_reduce<1 + 2 * kQrNB>(vals, sh, red);
+    const double xnorm2 = red[0];
+    const double2 alpha = pv[0];
+    double2 tj = make_double2(0.0, 0.0);
+    double2 scale = make_double2(1.0, 0.0);
+    double beta = alpha.x;
+    if (!(xnorm2 == 0.0 && alpha.y == 0.0)) {
+      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
+      beta = alpha.x >= 0.0 ? -nrm : nrm;
+      tj = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+      // scale = 1 / (alpha - beta)
+      const double dr = alpha.x - beta, di = alpha.y;
+      const double den = dr * dr + di * di;
+      scale = make_double2(dr / den, -di / den);
+    }
+    if (tid == 0) sTau[jj] = tj;
+    const bool refl = !(tj.x == 0.0 && tj.y == 0.0);
+    // f_c = conj(tau) (a_jc + conj(scale) g_c): coefficient of v in H^H a_c
+    double2 f[kQrNB];
+    const double2 ct = make_double2(tj.x, -tj.y);
+    const double2 cs = make_double2(scale.x, -scale.y);
+#pragma unroll
+    for (int c = 0; c < kQrNB; ++c)
+      f[c] = (refl && c > 0 && c < nc)
+                 ? cmul(ct, cadd(pv[c], cmul(cs, make_double2(red[1 + 2 * c], red[2 + 2 * c]))))
+                 : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = r0 + tid + kPanelThreads * i;
+      const double2 x = a[i][0];
+      double2 vr = (refl && r > j) ? cmul(x, scale) : x;  // v_r below the diagonal
+      double2 out = vr;                                   // final column value
+      if (r == j) {
+        out = make_double2(beta, 0.0);
+        vr = make_double2(1.0, 0.0);
+      }
+      if (r < j) out = x;
+      if (r < m) {
+        X[r + (int64_t)j * ld] = out;
+        Vall[r + (int64_t)j * m] = r < j ? make_double2(0.0, 0.0) : vr;
+      }
+      // later columns: a_c -= v f_c on rows >= j, then shift left
+#pragma unroll
+      for (int c = 1; c < kQrNB; ++c) {
+        double2 y = a[i][c];
+        if (r >= j) y = csub(y, cmul(vr, f[c]));
+        a[i][c - 1] = y;
+      }
+      a[i][kQrNB - 1] = make_double2(0.0, 0.0);
+    }
+  }
+}
+
+// The panel columns into registers (rows j0 + tid + 256 i, column c in slot c).
+template <int RPT>
+__device__ __forceinline__ void load_panel(double2 (&a)[RPT][kQrNB], const double2* __restrict__ X,
+                                           int m, int ld, int j0, int jb) {
+  const int tid = threadIdx.x;
 #pragma unroll
   for (int i = 0; i < RPT; ++i) {
     const int r = j0 + tid + kPanelThreads * i;
@@ -78,207 +247,141 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_qr(double2* __restrict_
     for (int c = 0; c < kQrNB; ++c)
       a[i][c] = (r < m && c < jb) ? X[r + (int64_t)(j0 + c) * ld] : make_double2(0.0, 0.0);
   }
-  double2 taus[kQrNB];
+}
+
+// sS[q][c] = v_q^H v_c for q < c, c in [C0, C1), v from the explicit V just
+// written (rows j0.. of columns j0..j0+jb-1; zero above the diagonal).
+template <int C0, int C1>
+__device__ __forceinline__ void panel_gram(const double2* __restrict__ Vall, int m, int j0, int jb,
+                                           double* sh, double* red, double2 (*sS)[kQrNB]) {
+  constexpr int NP = (C1 * (C1 - 1) - C0 * (C0 - 1)) / 2;  // pairs q < c, C0 <= c < C1
+  const int tid = threadIdx.x;
+  double s[2 * NP];
 #pragma unroll
-  for (int jj = 0; jj < kQrNB; ++jj) {
-    taus[jj] = make_double2(0.0, 0.0);
-    if (jj >= jb) continue;
-    const int j = j0 + jj;
-    // every thread has read the previous column's s_alpha
-    __syncthreads();
-    // ---- reflector for column jj: alpha = X[j][j], x = X[j+1:, j]
-    double part[1] = {0.0};
+  for (int t = 0; t < 2 * NP; ++t) s[t] = 0.0;
+#pragma unroll 1
+  for (int r = j0 + tid; r < m; r += kPanelThreads) {
+    double2 v[C1];
 #pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-      const int r = j0 + tid + kPanelThreads * i;
-      if (r > j && r < m) part[0] += a[i][jj].x * a[i][jj].x + a[i][jj].y * a[i][jj].y;
-    }
-    if (tid == jj) s_alpha = a[0][jj];
-    block_reduce<kPanelThreads>(part, 1, sh, red);
-    const double xnorm2 = red[0];
-    const double2 alpha = s_alpha;
-    double2 tj = make_double2(0.0, 0.0);
-    if (!(xnorm2 == 0.0 && alpha.y == 0.0)) {
-      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
-      const double beta = alpha.x >= 0.0 ? -nrm : nrm;
-      tj = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-      // scale = 1 / (alpha - beta)
-      const double dr = alpha.x - beta, di = alpha.y;
-      const double den = dr * dr + di * di;
-      const double2 scale = make_double2(dr / den, -di / den);
+    for (int c = 0; c < C1; ++c)
+      v[c] = c < jb ? Vall[r + (int64_t)(j0 + c) * m] : make_double2(0.0, 0.0);
+    int t = 0;
 #pragma unroll
-      for (int i = 0; i < RPT; ++i) {
-        const int r = j0 + tid + kPanelThreads * i;
-        if (r > j && r < m) a[i][jj] = cmul(a[i][jj], scale);
-      }
-      if (tid == jj) a[0][jj] = make_double2(beta, 0.0);
-    }
-    taus[jj] = tj;
-    // ---- apply H^H = I - conj(tau) v v^H to columns jj+1 .. jb-1
-    if (jj + 1 < jb && !(tj.x == 0.0 && tj.y == 0.0)) {
-      double w[2 * kQrNB];
+    for (int c = C0; c < C1; ++c) {
 #pragma unroll
-      for (int c = 0; c < 2 * kQrNB; ++c) w[c] = 0.0;
-#pragma unroll
-      for (int i = 0; i < RPT; ++i) {
-        const int r = j0 + tid + kPanelThreads * i;
-        if (r < j || r >= m) continue;
-        const double2 v = (r == j) ? make_double2(1.0, 0.0) : a[i][jj];
-#pragma unroll
-        for (int c = 0; c < kQrNB; ++c) {
-          if (c <= jj || c >= jb) continue;
-          const double2 p = cmulc(v, a[i][c]);
-          w[2 * c] += p.x;
-          w[2 * c + 1] += p.y;
-        }
-      }
-      block_reduce<kPanelThreads>(w, 2 * kQrNB, sh, red);
-      const double2 ct = make_double2(tj.x, -tj.y);  // conj(tau)
-#pragma unroll
-      for (int c = 0; c < kQrNB; ++c) {
-        if (c <= jj || c >= jb) continue;
-        const double2 f = cmul(ct, make_double2(red[2 * c], red[2 * c + 1]));
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-          const int r = j0 + tid + kPanelThreads * i;
-          if (r < j || r >= m) continue;
-          const double2 v = (r == j) ? make_double2(1.0, 0.0) : a[i][jj];
-          const double2 d = cmul(v, f);
-          a[i][c].x -= d.x;
-          a[i][c].y -= d.y;
-        }
+      for (int q = 0; q < c; ++q, ++t) {
+        const double2 p = cmulc(v[q], v[c]);
+        s[2 * t] += p.x;
+        s[2 * t + 1] += p.y;
       }
     }
   }
-  // ---- write back the panel and the explicit V
-#pragma unroll
-  for (int i = 0; i < RPT; ++i) {
-    const int r = j0 + tid + kPanelThreads * i;
-    if (r >= m) continue;
-#pragma unroll
-    for (int c = 0; c < kQrNB; ++c) {
-      if (c >= jb) continue;
-      X[r + (int64_t)(j0 + c) * ld] = a[i][c];
-      const int jr = j0 + c;
-      Vall[r + (int64_t)(j0 + c) * m] =
-          r < jr ? make_double2(0.0, 0.0) : (r == jr ? make_double2(1.0, 0.0) : a[i][c]);
-    }
-  }
-  // rows j0 .. j0+jb-1 above the panel rows of other threads are covered above;
-  // rows < j0 of Vall column are zero
-  for (int r = tid; r < j0; r += kPanelThreads)
-    for (int c = 0; c < jb; ++c) Vall[r + (int64_t)(j0 + c) * m] = make_double2(0.0, 0.0);
-  // ---- S[q][c] = v_q^H v_c for q < c (one reduction per column c, to keep
-  // the register footprint at the panel itself), then T (zlarft, forward,
-  // columnwise)
-  {
-#pragma unroll
-    for (int c = 1; c < kQrNB; ++c) {
-      if (c >= jb) continue;
-      double s[2 * kQrNB];
-#pragma unroll
-      for (int t = 0; t < 2 * kQrNB; ++t) s[t] = 0.0;
-#pragma unroll
-      for (int i = 0; i < RPT; ++i) {
-        const int r = j0 + tid + kPanelThreads * i;
-        if (r >= m || r < j0 + c) continue;  // v_c is zero above row j0+c
-        const double2 vc = (r == j0 + c) ? make_double2(1.0, 0.0) : a[i][c];
-#pragma unroll
-        for (int q = 0; q < kQrNB; ++q) {
-          if (q >= c) continue;
-          const double2 p = cmulc(a[i][q], vc);  // r > j0+q, so v_q[r] = a[i][q]
-          s[2 * q] += p.x;
-          s[2 * q + 1] += p.y;
-        }
-      }
-      block_reduce<kPanelThreads>(s, 2 * c, sh, red);
-      if (tid < c) sS[tid][c] = make_double2(red[2 * tid], red[2 * tid + 1]);
-    }
-    __syncthreads();
-    if (tid == 0) {
-      for (int q = 0; q < kQrNB; ++q)
-        for (int c = 0; c < kQrNB; ++c) sT[q][c] = make_double2(0.0, 0.0);
-      for (int c = 0; c < jb; ++c) {
-        const double2 tc = taus[c];
-        sT[c][c] = tc;
-        // T[0:c, c] = -tau_c T[0:c, 0:c] S[0:c, c]
-        for (int q = 0; q < c; ++q) {
-          double2 acc = make_double2(0.0, 0.0);
-          for (int p = q; p < c; ++p) acc = cadd(acc, cmul(sT[q][p], sS[p][c]));
-          const double2 v = cmul(tc, acc);
-          sT[q][c] = make_double2(-v.x, -v.y);
-        }
-      }
-    }
-    __syncthreads();
-    if (tid < kQrNB * kQrNB) {
-      const int q = tid / kQrNB, c = tid % kQrNB;
-      const double2 th = sT[c][q];  // (-T^H)[q][c] = -conj(T[c][q])
-      Tb[q * kQrNB + c] = make_double2(-th.x, th.y);
-      const double2 t = sT[q][c];
-      Tb[kQrNB * kQrNB + q * kQrNB + c] = make_double2(-t.x, -t.y);
-    }
-#pragma unroll
-    for (int c = 0; c < kQrNB; ++c)
-      if (tid == c && c < jb) tau[j0 + c] = taus[c];
+  block_reduce<2 * NP>(s, sh, red);
+  if (tid == 0) {
+    int t = 0;
+    for (int c = C0; c < C1; ++c)
+      for (int q = 0; q < c; ++q, ++t) sS[q][c] = make_double2(red[2 * t], red[2 * t + 1]);
   }
 }
 
-// A[j0:m, c] += V (F (V^H A[j0:m, c])) for columns c in [c_begin, c_end), with
-// V = Vall[:, j0:j0+jb] and F the row-major 8x8 factor (-T^H or -T).
-__global__ __launch_bounds__(kApplyThreads) void k_apply_wy(double2* __restrict__ A, int lda, int m,
-                                                            int j0, int jb, int c_begin, int c_end,
+// Apply the block reflector of panel jp (-T^H in LDS) to columns
+// c_begin + blockIdx.x (one workgroup per column).
+__global__ __launch_bounds__(kPanelThreads) void k_qr_apply(double2* __restrict__ X, int m, int ld,
+                                                            int jp, int jbp, int c_begin, int c_end,
                                                             const double2* __restrict__ Vall,
-                                                            const double2* __restrict__ F) {
-  __shared__ double sh[(kApplyThreads / 64) * 2 * kQrNB * kApplyCols];
-  __shared__ double red[2 * kQrNB * kApplyCols];
-  __shared__ double2 sW2[kQrNB][kApplyCols];
+                                                            const double2* __restrict__ Tall) {
+  __shared__ double sh[(kPanelThreads / 64) * 2 * kQrNB];
+  __shared__ double red[2 * kQrNB];
+  __shared__ double2 sF[kQrNB * kQrNB];
+  __shared__ double2 sG[kQrNB];
+  const int c = c_begin + (int)blockIdx.x;
+  if (c >= c_end) return;
+  if (threadIdx.x < kQrNB * kQrNB) sF[threadIdx.x] = Tall[(size_t)(jp / kQrNB) * 2 * kQrNB * kQrNB + threadIdx.x];
+  lds_barrier();
+  apply_prev_column(X, m, ld, c, jp, jbp, Vall, sF, sh, red, sG);
+}
+
+// One step of the R phase: workgroup 0 factors panel [j0, j0+jb) (already
+// updated by all previous reflectors); workgroups 1.. apply the reflector of
+// panel jp (jp < 0: none) to columns j0+jb, j0+jb+1, ..., n-1.
+// Outputs: R / v in X, tau[j0..), explicit V columns in Vall (zeros above
+// the unit diagonal) and Tall[p*128 ..] = -T^H (row-major 8x8) followed by -T.
+template <int RPT>
+__global__ __launch_bounds__(kPanelThreads) void k_qr_step(double2* __restrict__ X, int m, int ld,
+                                                           int n, int jp, int jbp, int j0, int jb,
+                                                           double2* __restrict__ tau,
+                                                           double2* __restrict__ Vall,
+                                                           double2* __restrict__ Tall) {
+  __shared__ double sh[(kPanelThreads / 64) * 64];
+  __shared__ double red[64];
+  __shared__ double2 sT[kQrNB][kQrNB];
+  __shared__ double2 sS[kQrNB][kQrNB];
+  __shared__ double2 sTau[kQrNB];
+  __shared__ double2 sPiv[2][kQrNB];
+  __shared__ double2 sF[kQrNB * kQrNB];
+  __shared__ double2 sG[kQrNB];
   const int tid = threadIdx.x;
-  const int c0 = c_begin + blockIdx.x * kApplyCols;
-  const int nc = min(kApplyCols, c_end - c0);
-  double w[2 * kQrNB * kApplyCols];
-#pragma unroll
-  for (int t = 0; t < 2 * kQrNB * kApplyCols; ++t) w[t] = 0.0;
-  for (int r = j0 + tid; r < m; r += kApplyThreads) {
-    double2 v[kQrNB];
-#pragma unroll
+  if (blockIdx.x > 0) {
+    const int c = j0 + jb + (int)blockIdx.x - 1;
+    if (c >= n || jp < 0) return;
+    if (tid < kQrNB * kQrNB) sF[tid] = Tall[(size_t)(jp / kQrNB) * 2 * kQrNB * kQrNB + tid];
+    lds_barrier();
+    apply_prev_column(X, m, ld, c, jp, jbp, Vall, sF, sh, red, sG);
+    return;
+  }
+  if (jb == 0) return;
+  double2 a[RPT][kQrNB];
+  load_panel<RPT>(a, X, m, ld, j0, jb);
+  factor_panel<RPT>(a, j0, m, ld, j0, jb, X, Vall, sh, red, sPiv, sTau);
+  for (int r = tid; r < j0; r += kPanelThreads)
+    for (int c = 0; c < jb; ++c) Vall[r + (int64_t)(j0 + c) * m] = make_double2(0.0, 0.0);
+  // ---- S[q][c] = v_q^H v_c (q < c) from the explicit V just written (each
+  // thread reads back its own rows), in two reductions; then T (zlarft,
+  // forward, columnwise) by thread 0
+  panel_gram<1, 5>(Vall, m, j0, jb, sh, red, sS);
+  panel_gram<5, kQrNB>(Vall, m, j0, jb, sh, red, sS);
+  lds_barrier();
+  if (tid == 0) {
     for (int q = 0; q < kQrNB; ++q)
-      v[q] = q < jb ? Vall[r + (int64_t)(j0 + q) * m] : make_double2(0.0, 0.0);
-#pragma unroll
-    for (int cc = 0; cc < kApplyCols; ++cc) {
-      if (cc >= nc) continue;
-      const double2 x = A[r + (int64_t)(c0 + cc) * lda];
-#pragma unroll
-      for (int q = 0; q < kQrNB; ++q) {
-        const double2 p = cmulc(v[q], x);
-        w[2 * (q * kApplyCols + cc)] += p.x;
-        w[2 * (q * kApplyCols + cc) + 1] += p.y;
+      for (int c = 0; c < kQrNB; ++c) sT[q][c] = make_double2(0.0, 0.0);
+    for (int c = 0; c < jb; ++c) {
+      const double2 tc = sTau[c];
+      sT[c][c] = tc;
+      // T[0:c, c] = -tau_c T[0:c, 0:c] S[0:c, c]
+      for (int q = 0; q < c; ++q) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int p = q; p < c; ++p) acc = cadd(acc, cmul(sT[q][p], sS[p][c]));
+        const double2 v = cmul(tc, acc);
+        sT[q][c] = make_double2(-v.x, -v.y);
       }
     }
   }
-  block_reduce<kApplyThreads>(w, 2 * kQrNB * kApplyCols, sh, red);
-  if (tid < kQrNB * kApplyCols) {
-    const int q = tid / kApplyCols, cc = tid % kApplyCols;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int p = 0; p < jb; ++p)
-      acc = cadd(acc, cmul(F[q * kQrNB + p], make_double2(red[2 * (p * kApplyCols + cc)],
-                                                         red[2 * (p * kApplyCols + cc) + 1])));
-    sW2[q][cc] = acc;
+  lds_barrier();
+  double2* Tb = Tall + (size_t)(j0 / kQrNB) * 2 * kQrNB * kQrNB;
+  if (tid < kQrNB * kQrNB) {
+    const int q = tid / kQrNB, c = tid % kQrNB;
+    const double2 th = sT[c][q];  // (-T^H)[q][c] = -conj(T[c][q])
+    Tb[q * kQrNB + c] = make_double2(-th.x, th.y);
+    const double2 t = sT[q][c];
+    Tb[kQrNB * kQrNB + q * kQrNB + c] = make_double2(-t.x, -t.y);
   }
-  __syncthreads();
-  for (int r = j0 + tid; r < m; r += kApplyThreads) {
-    double2 v[kQrNB];
-#pragma unroll
-    for (int q = 0; q < kQrNB; ++q)
-      v[q] = q < jb ? Vall[r + (int64_t)(j0 + q) * m] : make_double2(0.0, 0.0);
-#pragma unroll
-    for (int cc = 0; cc < kApplyCols; ++cc) {
-      if (cc >= nc) continue;
-      double2 x = A[r + (int64_t)(c0 + cc) * lda];
-#pragma unroll
-      for (int q = 0; q < kQrNB; ++q) x = cadd(x, cmul(v[q], sW2[q][cc]));
-      A[r + (int64_t)(c0 + cc) * lda] = x;
-    }
+  if (tid < jb) tau[j0 + tid] = sTau[tid];
+}
+
+// N (kp x kp, row-major) <- 0 except the per-panel diagonal blocks -T_p;
+// S (kp x kp) <- 0 (the Gram GEMM fills its k x k corner)
+__global__ void k_qr_tinit(double2* __restrict__ N, double2* __restrict__ S, int kp, int np,
+                           const double2* __restrict__ Tall) {
+  const int64_t total = (int64_t)kp * kp;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t / kp), j = (int)(t - (int64_t)i * kp);
+    const int p = i / kQrNB;
+    double2 v = make_double2(0.0, 0.0);
+    if (p < np && j / kQrNB == p)
+      v = Tall[(size_t)p * 2 * kQrNB * kQrNB + kQrNB * kQrNB + (i % kQrNB) * kQrNB + (j % kQrNB)];
+    N[t] = v;
+    S[t] = make_double2(0.0, 0.0);
   }
 }
 
@@ -291,77 +394,149 @@ __global__ void k_eye(double2* Q, int m, int k) {
   }
 }
 
-size_t qr_ws_bytes(int m, int n) {
-  const int k = m < n ? m : n;
-  const int np = (k + kQrNB - 1) / kQrNB;
-  return align_up((size_t)m * k * 16) /* Vall */ + align_up((size_t)(k + 1) * 16) /* tau */ +
-         align_up((size_t)np * 2 * kQrNB * kQrNB * 16) /* T factors */;
+struct QrLayout {
+  int k, np, kp;
+  size_t off_V, off_tau, off_T, off_N, off_S, off_Y, off_X, off_part, part_bytes, total;
+};
+
+static QrLayout qr_layout(int m, int n) {
+  QrLayout L{};
+  L.k = m < n ? m : n;
+  L.np = (L.k + kQrNB - 1) / kQrNB;
+  L.kp = kQrNB;
+  while (L.kp < L.np * kQrNB) L.kp *= 2;
+  const int k = L.k, kp = L.kp;
+  size_t part = 0;
+  part = std::max(part, splitk_bytes(RENO_PREC_FP64, k, k, m, 1));  // Gram
+  part = std::max(part, splitk_bytes(RENO_PREC_FP64, k, k, k, 1));  // X
+  part = std::max(part, splitk_bytes(RENO_PREC_FP64, k, m, k, 1));  // Q
+  for (int b = kQrNB; b < kp; b *= 2) part = std::max(part, splitk_bytes(RENO_PREC_FP64, b, b, b, kp / (2 * b)));
+  size_t o = 0;
+  L.off_V = o; o += align_up((size_t)m * k * 16);
+  L.off_tau = o; o += align_up((size_t)(k + 1) * 16);
+  L.off_T = o; o += align_up((size_t)L.np * 2 * kQrNB * kQrNB * 16);
+  L.off_N = o; o += align_up((size_t)kp * kp * 16);
+  L.off_S = o; o += align_up((size_t)kp * kp * 16);
+  L.off_Y = o; o += align_up((size_t)kp * kp / 4 * 16 + 16);
+  L.off_X = o; o += align_up((size_t)k * k * 16);
+  L.off_part = o; o += align_up(part);
+  L.part_bytes = align_up(part);
+  L.total = o;
+  return L;
 }
+
+size_t qr_ws_bytes(int m, int n) { return qr_layout(m, n).total; }
 
 bool qr_supported(int m, int n) {
   (void)n;
-  return m <= 5 * kPanelThreads;
+  return m <= kMaxRpt * kPanelThreads;
 }
 
 template <int RPT>
-static void launch_panel(hipStream_t s, double2* X, int m, int j0, int jb, double2* tau,
-                         double2* Vall, double2* Tb) {
-  hipLaunchKernelGGL(k_panel_qr<RPT>, dim3(1), dim3(kPanelThreads), 0, s, X, m, m, j0, jb, tau,
-                     Vall, Tb);
+static void launch_step(hipStream_t s, int grid, double2* X, int m, int n, int jp, int jbp, int j0,
+                        int jb, double2* tau, double2* Vall, double2* Tall) {
+  hipLaunchKernelGGL(k_qr_step<RPT>, dim3(grid), dim3(kPanelThreads), 0, s, X, m, m, n, jp, jbp, j0,
+                     jb, tau, Vall, Tall);
+}
+
+static GemmCall zgemm_call(int M, int N, int K, int batch, const void* A, int64_t sAm, int64_t sAk,
+                           int64_t sAb, int conjA, const void* B, int64_t sBk, int64_t sBn,
+                           int64_t sBb, int conjB, void* C, int64_t sCm, int64_t sCb, int beta,
+                           void* part, size_t part_bytes) {
+  GemmCall g{};
+  g.M = M; g.N = N; g.K = K; g.batch = batch; g.prec = RENO_PREC_FP64;
+  g.A = A; g.a_kind = ST_Z128; g.sAm = sAm; g.sAk = sAk; g.sAb = sAb; g.conjA = conjA;
+  g.B = B; g.b_kind = ST_Z128; g.sBk = sBk; g.sBn = sBn; g.sBb = sBb; g.conjB = conjB;
+  g.C = C; g.c_kind = ST_Z128; g.sCm = sCm; g.sCn = 1; g.sCb = sCb; g.beta = beta;
+  g.part = part; g.part_bytes = part_bytes;
+  return g;
 }
 
 int qr_householder(hipStream_t s, int m, int n, double2* X, double2* Q, void* ws) {
-  const int k = m < n ? m : n;
+  const QrLayout L = qr_layout(m, n);
+  const int k = L.k;
   if (k <= 0) return RENO_OK;
   if (!qr_supported(m, n)) {
     set_error("qr_householder: %d rows exceed the register-resident panel", m);
     return RENO_EINVAL;
   }
-  const int np = (k + kQrNB - 1) / kQrNB;
-  char* p = (char*)ws;
-  double2* Vall = (double2*)p;
-  p += align_up((size_t)m * k * 16);
-  double2* tau = (double2*)p;
-  p += align_up((size_t)(k + 1) * 16);
-  double2* Tall = (double2*)p;
-  for (int ip = 0; ip < np; ++ip) {
-    const int j0 = ip * kQrNB;
-    const int jb = (k - j0) < kQrNB ? (k - j0) : kQrNB;
-    const int rpt = (m - j0 + kPanelThreads - 1) / kPanelThreads;
-    double2* Tb = Tall + (size_t)ip * 2 * kQrNB * kQrNB;
-    switch (rpt) {
-      case 1: launch_panel<1>(s, X, m, j0, jb, tau, Vall, Tb); break;
-      case 2: launch_panel<2>(s, X, m, j0, jb, tau, Vall, Tb); break;
-      case 3: launch_panel<3>(s, X, m, j0, jb, tau, Vall, Tb); break;
-      case 4: launch_panel<4>(s, X, m, j0, jb, tau, Vall, Tb); break;
-      default: launch_panel<5>(s, X, m, j0, jb, tau, Vall, Tb); break;
-    }
-    RENO_CHECK_LAUNCH();
-    const int cb = j0 + jb;
-    if (cb < n) {
-      const int blocks = (n - cb + kApplyCols - 1) / kApplyCols;
-      hipLaunchKernelGGL(k_apply_wy, dim3(blocks), dim3(kApplyThreads), 0, s, X, m, m, j0, jb, cb,
-                         n, (const double2*)Vall, (const double2*)Tb);
+  char* base = (char*)ws;
+  double2* Vall = (double2*)(base + L.off_V);
+  double2* tau = (double2*)(base + L.off_tau);
+  double2* Tall = (double2*)(base + L.off_T);
+  double2* Nm = (double2*)(base + L.off_N);
+  double2* Sm = (double2*)(base + L.off_S);
+  double2* Y = (double2*)(base + L.off_Y);
+  double2* Xm = (double2*)(base + L.off_X);
+  void* part = base + L.off_part;
+  // ---- R phase: per panel, the previous reflector on the panel columns,
+  // then the panel factorisation alongside the rest of the trailing update;
+  // a final launch brings the last reflector to columns k..n-1
+  for (int ip = 0; ip <= L.np; ++ip) {
+    const int j0 = ip < L.np ? ip * kQrNB : k;
+    const int jb = ip < L.np ? std::min(kQrNB, k - j0) : 0;
+    const int jp = ip > 0 ? (ip - 1) * kQrNB : -1;
+    const int jbp = ip > 0 ? std::min(kQrNB, k - jp) : 0;
+    const int trail = ip > 0 ? std::max(0, n - (j0 + jb)) : 0;
+    if (jb == 0 && trail == 0) break;
+    if (jp >= 0 && jb > 0) {
+      hipLaunchKernelGGL(k_qr_apply, dim3(jb), dim3(kPanelThreads), 0, s, X, m, m, jp, jbp, j0, j0 + jb,
+                         (const double2*)Vall, (const double2*)Tall);
       RENO_CHECK_LAUNCH();
     }
+    const int rpt = std::max(1, (m - j0 + kPanelThreads - 1) / kPanelThreads);
+    const int grid = 1 + trail;
+    switch (rpt) {
+      case 1: launch_step<1>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
+      case 2: launch_step<2>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
+      case 3: launch_step<3>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
+      case 4: launch_step<4>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
+      case 5: launch_step<5>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
+      case 6: launch_step<6>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
+      case 7: launch_step<7>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
+      case 8: launch_step<8>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
+      case 9: launch_step<9>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
+      default: launch_step<10>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
+    }
+    RENO_CHECK_LAUNCH();
   }
-  // Q = H_0 ... H_{k-1} applied to the first k columns of I (zungqr)
+  // ---- Q phase
+  const int kp = L.kp;
+  {
+    const int64_t tot = (int64_t)kp * kp;
+    const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_qr_tinit, dim3(blocks), dim3(256), 0, s, Nm, Sm, kp, L.np,
+                       (const double2*)Tall);
+    RENO_CHECK_LAUNCH();
+  }
+  if (L.np > 1) {
+    // Gram S = V^H V (k x k corner of the kp x kp buffer)
+    RENO_TRY(launch_gemm(zgemm_call(k, k, m, 1, Vall, m, 1, 0, 1, Vall, 1, m, 0, 0, Sm, kp, 0, 0,
+                                    part, L.part_bytes), s));
+    // recursive doubling: N12 = N11 S12 N22 for adjacent b x b blocks
+    for (int b = kQrNB; b < kp; b *= 2) {
+      const int pairs = kp / (2 * b);
+      const int64_t bs = (int64_t)2 * b * (kp + 1);
+      // Y_i = S[o1:, o2:] N[o2:, o2:]
+      RENO_TRY(launch_gemm(zgemm_call(b, b, b, pairs, Sm + b, kp, 1, bs, 0, Nm + (int64_t)b * kp + b, kp, 1,
+                                      bs, 0, Y, b, (int64_t)b * b, 0, part, L.part_bytes), s));
+      // N[o1:, o2:] = N[o1:, o1:] Y_i
+      RENO_TRY(launch_gemm(zgemm_call(b, b, b, pairs, Nm, kp, 1, bs, 0, Y, b, 1, (int64_t)b * b, 0,
+                                      Nm + b, kp, bs, 0, part, L.part_bytes), s));
+    }
+  }
+  // X = N V1^H  (k x k)
+  RENO_TRY(launch_gemm(zgemm_call(k, k, k, 1, Nm, kp, 1, 0, 0, Vall, m, 1, 0, 1, Xm, k, 0, 0, part,
+                                  L.part_bytes), s));
+  // Q = E + V X, computed as Q^T = E^T + X^T V^T into the column-major Q
   {
     const int64_t tot = (int64_t)m * k;
-    const int blocks = (int)((tot + 255) / 256 < 4096 ? (tot + 255) / 256 : 4096);
+    const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 4096);
     hipLaunchKernelGGL(k_eye, dim3(blocks), dim3(256), 0, s, Q, m, k);
     RENO_CHECK_LAUNCH();
   }
-  for (int ip = np - 1; ip >= 0; --ip) {
-    const int j0 = ip * kQrNB;
-    const int jb = (k - j0) < kQrNB ? (k - j0) : kQrNB;
-    double2* Tb = Tall + (size_t)ip * 2 * kQrNB * kQrNB;
-    // Q[j0:, j0:k] <- (I - V T V^H) Q[j0:, j0:k]  (factor -T in the second slot)
-    const int blocks = (k - j0 + kApplyCols - 1) / kApplyCols;
-    hipLaunchKernelGGL(k_apply_wy, dim3(blocks), dim3(kApplyThreads), 0, s, Q, m, m, j0, jb, j0, k,
-                       (const double2*)Vall, (const double2*)(Tb + kQrNB * kQrNB));
-    RENO_CHECK_LAUNCH();
-  }
+  RENO_TRY(launch_gemm(zgemm_call(k, m, k, 1, Xm, 1, k, 0, 0, Vall, m, 1, 0, 0, Q, m, 0, 1, part,
+                                  L.part_bytes), s));
   return RENO_OK;
 }
 
