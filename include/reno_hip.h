@@ -146,6 +146,22 @@ int reno_block_qr(int system, int rows, int cols, const void* coef,
                   int nblocks, const reno_qn_block* blocks, void* u, void* v,
                   int v_transposed, void* ws, size_t ws_bytes, void* stream);
 
+/* ---- DMRG local eigenproblem (FP64: L/R complex128, W float64) --------
+ * Diagonal of the one-site H_eff, the Davidson preconditioner (replaces the
+ * hdiag contraction of gs.py:424-436):
+ *   out[a,d,l] = sum_{b,f} L[a,b,a] W[b,d,d,f] R[l,f,l]      (complex128)
+ * Two-site problems pass the two MPO sites merged into one (d = d0*d1). */
+int reno_heff_diag(int Ml, int d, int Mr, int Dl, int Dr, const void* L,
+                   const void* W, const void* R, void* out, void* stream);
+/* Dense H_eff on the allowed quantum-number entries, for the direct
+ * diagonalisation of small sites (replaces get_ham_direct, gs.py:302-330):
+ *   out[i][j] = sum_{b,f} L[a,b,c] W[b,d,e,f] R[l,f,k],
+ *   (a,d,l) = idx[i], (c,e,k) = idx[j]  (flat row-major site indices;
+ *   idx is a device int64 array of nidx entries; out nidx x nidx complex128) */
+int reno_heff_dense(int Ml, int d, int Mr, int Dl, int Dr, const void* L,
+                    const void* W, const void* R, const int64_t* idx, int nidx,
+                    void* out, void* stream);
+
 /* ---- tensordot (replaces matrix.py:210-211 for the 2-D contractions of
  * the sweep: absorbing the evolved bond into the neighbouring site) ------
  * C (M x N) = A (M x K) . B (K x N), all row-major complex128. */

@@ -1,0 +1,498 @@
+"""DMRG ground state and state-averaged low-lying states (reference
+mps/gs.py:54-576) with the local eigenproblem on the device.
+
+Same sweep as the reference: one- or two-site updates along the chain, the
+``procedure`` of OptimizeConfig giving the bond dimension and the
+``percent`` of renormalised states shared equally among quantum-number
+sectors (select_basis, J. Chem. Phys. 120, 3172 (2004)); convergence when the
+two lowest sweep energies agree within e_rtol / e_atol after the
+percent-free sweeps begin.
+
+Device parts: environments (native.env_update), the effective-Hamiltonian
+matvec (native.hop_apply; two-site problems merge the two MPO sites into one
+of physical dimension d0*d1), its diagonal and, for sites with fewer than
+1000 allowed coefficients, its dense form on the allowed entries
+(native.heff_diag / native.heff_dense), the Davidson vectors, and the
+site updates.  The host keeps the quantum-number bookkeeping, the Davidson
+subspace matrix (at most a few dozen rows), the basis selection, and - as
+the reference does (gs.py:397-398, mp.py:693) - the small dense
+eigenproblems and the blocked SVD / density-matrix eigendecompositions of
+the basis update (LAPACK; at most a few hundred rows per block).
+
+Not on this path: the ``omega`` excited-state target ((H - omega)^2 with a
+two-layer MPO), StackedMpo, on-the-fly site swapping and the primme solver.
+"""
+import logging
+from typing import List, Union
+
+import numpy as np
+import scipy.linalg
+import torch
+
+from renormalizer_amd import native
+from renormalizer_amd.mps.lib import Environ
+from renormalizer_amd.mps.mpo import Mpo
+from renormalizer_amd.mps.mps import Mps
+from renormalizer_amd.mps.svd_qn import get_qn_mask, optimized_svd, qn_blocks, select_basis
+from renormalizer_amd.utils import CompressConfig, CompressCriteria, Quantity
+
+logger = logging.getLogger(__name__)
+
+
+def construct_mps_mpo(model, mmax, nexciton, offset=Quantity(0)):
+    """MPO of the model and a random MPS in the ``nexciton`` sector
+    (reference gs.py:31-51)."""
+    mpo = Mpo(model, offset=offset)
+    mps = Mps.random(model, nexciton, mmax, percent=1)
+    return mps, mpo
+
+
+# ---------------------------------------------------------------------------
+# Davidson (Davidson-Liu with the diagonal preconditioner), device vectors
+
+
+def davidson(aop, x0: List[torch.Tensor], precond, tol=1e-12, max_cycle=100, max_space=12,
+             lindep=1e-14, nroots=1):
+    """Lowest ``nroots`` eigenpairs of the Hermitian operator ``aop`` from the
+    guesses ``x0`` (flat device vectors).  The subspace grows by the
+    preconditioned residuals of the unconverged roots and restarts from the
+    Ritz vectors when it would exceed max_space + 3 (nroots - 1); a root is
+    converged when its Ritz value moved by less than ``tol`` and its residual
+    norm is below sqrt(tol) (the criteria and defaults of the reference's
+    solver, lib/davidson/davidson.py:154-441 and gs.py:536-538).
+    Returns (e, x): e a float (nroots == 1) or an array, x a list of vectors."""
+    toloose = np.sqrt(tol)
+    max_space = max_space + (nroots - 1) * 3
+    V: List[torch.Tensor] = []
+    AV: List[torch.Tensor] = []
+
+    def orth_add(xs):
+        added = 0
+        for x in xs:
+            x = x.clone()
+            for _ in range(2):  # two Gram-Schmidt passes
+                if V:
+                    Vt = torch.stack(V)
+                    x = x - Vt.T @ (Vt.conj() @ x)
+            nrm = torch.linalg.vector_norm(x).item()
+            if nrm * nrm < lindep:
+                continue
+            x = x / nrm
+            V.append(x)
+            AV.append(aop(x))
+            added += 1
+        return added
+
+    orth_add(x0)
+    if not V:
+        raise RuntimeError("no linearly independent Davidson guess")
+    e_prev = None
+    e = X = None
+    for _ in range(max_cycle):
+        Vt = torch.stack(V)
+        AVt = torch.stack(AV)
+        h = (Vt.conj() @ AVt.T).cpu().numpy()
+        h = 0.5 * (h + h.conj().T)
+        w, u = scipy.linalg.eigh(h)
+        k = min(nroots, len(w))
+        e = w[:k]
+        uc = torch.from_numpy(np.ascontiguousarray(u[:, :k])).to(Vt.device).to(Vt.dtype)
+        X = (uc.T @ Vt)              # (k, n) Ritz vectors
+        AX = (uc.T @ AVt)
+        Rv = AX - torch.from_numpy(e).to(Vt.device).to(Vt.dtype)[:, None] * X
+        rnorm = torch.linalg.vector_norm(Rv, dim=1).cpu().numpy()
+        de = np.full(k, np.inf) if e_prev is None or len(e_prev) != k else np.abs(e - e_prev)
+        conv = (de < tol) & (rnorm < toloose)
+        if conv.all():
+            break
+        e_prev = e
+        new = [precond(Rv[i], e[i]) for i in range(k) if not conv[i]]
+        if len(V) + len(new) > max_space:
+            V[:] = [X[i] for i in range(k)]
+            AV[:] = [AX[i] for i in range(k)]
+        if orth_add(new) == 0:
+            break
+    xs = [X[i] for i in range(X.shape[0])]
+    if nroots == 1:
+        return float(e[0]), xs
+    return np.array(e), xs
+
+
+def _sign_fix(v: torch.Tensor) -> torch.Tensor:
+    """Phase making the largest-magnitude component real positive (the
+    reference's sign_fix, gs.py:353-361, for real vectors)."""
+    i = int(torch.argmax(v.abs()).item())
+    ph = v[i] / v[i].abs()
+    return v / ph
+
+
+# ---------------------------------------------------------------------------
+# the local problem
+
+
+def _merge_w(w0: np.ndarray, w1: np.ndarray) -> np.ndarray:
+    """Two MPO sites as one: W[b, (d g), (e h), j] = sum_f W0[b,d,e,f] W1[f,g,h,j]."""
+    m = np.einsum("bdef,fghj->bdgehj", w0, w1)
+    Dl, d0, d1, Dr = w0.shape[0], w0.shape[1], w1.shape[1], w1.shape[3]
+    return np.ascontiguousarray(m.reshape(Dl, d0 * d1, d0 * d1, Dr))
+
+
+def _local_problem(mps, mpo, cidx, dev):
+    w = [np.ascontiguousarray(mpo[i], dtype=np.float64) for i in cidx]
+    wm = w[0] if len(w) == 1 else _merge_w(w[0], w[1])
+    return torch.from_numpy(wm).to(dev).contiguous()
+
+
+def _eigh_local(mps, ltensor, rtensor, W, qn_mask, guesses, nroots, inverse, use_direct):
+    cshape = qn_mask.shape
+    Ml, Mr = cshape[0], cshape[-1]
+    d = int(np.prod(cshape[1:-1]))
+    shape3 = (Ml, d, Mr)
+    dev = ltensor.device
+    flat_mask = qn_mask.reshape(-1)
+    if use_direct:
+        idx = torch.from_numpy(np.flatnonzero(flat_mask).astype(np.int64)).to(dev)
+        ham = (native.heff_dense(ltensor, W, rtensor, shape3, idx) * inverse).cpu().numpy()
+        if not np.isfinite(ham).all():
+            raise FloatingPointError("non-finite effective Hamiltonian")
+        # small dense eigenproblem on the host (LAPACK), as the reference does
+        w, v = scipy.linalg.eigh(0.5 * (ham + ham.conj().T))
+        k = min(nroots, v.shape[1])
+        vd = torch.from_numpy(np.ascontiguousarray(v[:, :k])).to(dev)
+        cs = []
+        for i in range(k):
+            full = torch.zeros(flat_mask.size, dtype=torch.complex128, device=dev)
+            full[idx] = _sign_fix(vd[:, i])
+            cs.append(full)
+        e = float(w[0]) if nroots == 1 else w[:k]
+        return e, cs, 0
+    mask_t = torch.from_numpy(flat_mask).to(dev)
+    hdiag = (native.heff_diag(ltensor, W, rtensor, shape3).reshape(-1) * inverse).real
+    count = [0]
+
+    def aop(x):
+        count[0] += 1
+        y = native.hop_apply(ltensor, W, rtensor, x.reshape(shape3), native.PREC_FP64).reshape(-1)
+        return (y * inverse) * mask_t
+
+    def precond(r, e):
+        den = hdiag - e + 1e-4
+        return torch.where(mask_t, r / den, torch.zeros_like(r))
+
+    e, xs = davidson(aop, [g * mask_t for g in guesses], precond, max_cycle=100, nroots=nroots)
+    return e, [_sign_fix(x) for x in xs], count[0]
+
+
+# ---------------------------------------------------------------------------
+# the MPS update (reference mp.py:651-888, SVD and state-averaged branches)
+
+
+def _blocked_svd(cmat, qnbigl, qnbigr, qntot):
+    """svd_qn with full_matrices=True (reference svd_qn.py:99-240): per
+    quantum-number block (reference block order) A_b = U S Vt, returned as
+    U (rows x .), S_U, qn_U, V (cols x .) with V = Vt^T, S_V, qn_V; each side
+    lists the singular vectors of all blocks, then the zero-weight
+    completions.  The blocks are small: LAPACK on the host, as the reference
+    (mp.py:693); the factors go back to the device."""
+    a = cmat.cpu().numpy()
+    if not np.isfinite(a).all():
+        raise FloatingPointError("non-finite coefficient matrix")
+    rows, cols = a.shape
+    u_l, u0_l, v_l, v0_l, s_l, su0, sv0 = [], [], [], [], [], [], []
+    qnl, qnl0, qnr, qnr0 = [], [], [], []
+    for nl, nr, lset, rset in qn_blocks(qnbigl, qnbigr, qntot):
+        bu, bs, bvt = optimized_svd(a[np.ix_(lset, rset)], full_matrices=True)
+        dim = bs.shape[0]
+        s_l.append(bs)
+        for src, lst, lst0, ql, ql0, sl0, ix, n, size in (
+                (bu, u_l, u0_l, qnl, qnl0, su0, lset, nl, rows),
+                (bvt.T, v_l, v0_l, qnr, qnr0, sv0, rset, nr, cols)):
+            rec = np.zeros((size, dim), dtype=a.dtype)
+            rec[ix] = src[:, :dim]
+            lst.append(rec)
+            ql += [tuple(n)] * dim
+            extra = src.shape[1] - dim
+            rec0 = np.zeros((size, extra), dtype=a.dtype)
+            rec0[ix] = src[:, dim:]
+            lst0.append(rec0)
+            ql0 += [tuple(n)] * extra
+            sl0.append(np.zeros(extra))
+    if not u_l:
+        raise ValueError("Invalid quantum number")
+    dev = cmat.device
+    U = torch.from_numpy(np.concatenate(u_l + u0_l, axis=1)).to(dev)
+    V = torch.from_numpy(np.concatenate(v_l + v0_l, axis=1)).to(dev)
+    SU = np.concatenate(s_l + su0)
+    SV = np.concatenate(s_l + sv0)
+    return U, SU, qnl + qnl0, V, SV, qnr + qnr0
+
+
+def _blocked_dm_eigh(ddm, qnbig, comp_qnbig, qntot):
+    """eigh_qn (reference svd_qn.py:243-303): per sector of the reduced
+    density matrix, eigenvectors and sqrt of the (clipped) eigenvalues
+    (host LAPACK; eigenvectors back on the device)."""
+    qn_size = len(qntot)
+    localqn = qnbig.reshape(-1, qn_size)
+    dm = ddm.cpu().numpy()
+    u_l, s_l, new_qn = [], [], []
+    for nl in set([tuple(t) for t in localqn]):
+        nr = qntot - np.array(nl)
+        if np.sum(get_qn_mask(comp_qnbig, nr)) == 0:
+            continue
+        lset = np.where(get_qn_mask(localqn, nl))[0]
+        block = dm[np.ix_(lset, lset)]
+        s2, bu = scipy.linalg.eigh(0.5 * (block + block.conj().T))
+        s2[s2 < 0] = 0
+        s_l.append(np.sqrt(s2))
+        rec = np.zeros((len(localqn), len(lset)), dtype=dm.dtype)
+        rec[lset] = bu
+        u_l.append(rec)
+        new_qn += [nl] * len(lset)
+    return torch.from_numpy(np.concatenate(u_l, axis=1)).to(ddm.device), np.concatenate(s_l), new_qn
+
+
+def _take_columns(mat, sidx, scale=None):
+    """Columns sidx of mat (times scale[p] for output column p); an index past
+    the last column gives a zero column (the reference's select_basis keeps
+    zero-weight completions of the other side that way)."""
+    out = torch.zeros((mat.shape[0], len(sidx)), dtype=mat.dtype, device=mat.device)
+    pos = [p for p, i in enumerate(sidx) if i < mat.shape[1]]
+    if pos:
+        src = torch.tensor([sidx[p] for p in pos], dtype=torch.int64, device=mat.device)
+        cols = mat.index_select(1, src)
+        if scale is not None:
+            cols = cols * torch.tensor([scale[p] for p in pos], dtype=torch.float64, device=mat.device)
+        out[:, torch.tensor(pos, dtype=torch.int64, device=mat.device)] = cols
+    return out
+
+
+def _update_mps(mps: Mps, cstruct, cidx, qnbigl, qnbigr, percent):
+    """Put the optimised coefficient(s) back into the MPS (reference
+    mp.py:651-888): SVD + basis selection for one root, the averaged reduced
+    density matrix for several.  Returns the rotated coefficients of every
+    root for the next site's guesses (state-averaged) or None."""
+    if mps.compress_config.bonddim_should_set:
+        mps.compress_config.set_bonddim(len(mps) + 1)
+    lshape = list(qnbigl.shape[:-1])
+    rshape = list(qnbigr.shape[:-1])
+    rows, cols = int(np.prod(lshape)), int(np.prod(rshape))
+    multi = isinstance(cstruct, list)
+    if not multi:
+        cmat = cstruct.reshape(rows, cols)
+        U, SU, qnl, V, SV, qnr = _blocked_svd(cmat, qnbigl, qnbigr, mps.qntot)
+        if mps.to_right:
+            m_trunc = mps.compress_config.compute_m_trunc(SU, cidx[0], mps.to_right)
+            sidx, msqn, _ = select_basis(None, SU, qnl, None, m_trunc, percent=percent)
+            ms = _take_columns(U, sidx).reshape(lshape + [len(sidx)])
+            comp = _take_columns(V, sidx, scale=[SU[i] for i in sidx])
+            compms = torch.movedim(comp.reshape(rshape + [len(sidx)]), -1, 0)
+        else:
+            m_trunc = mps.compress_config.compute_m_trunc(SV, cidx[-1], mps.to_right)
+            sidx, msqn, _ = select_basis(None, SV, qnr, None, m_trunc, percent=percent)
+            ms = torch.movedim(_take_columns(V, sidx).reshape(rshape + [len(sidx)]), -1, 0)
+            compms = _take_columns(U, sidx, scale=[SV[i] for i in sidx]).reshape(lshape + [len(sidx)])
+        rotated = None
+    else:
+        mats = [c.reshape(rows, cols) for c in cstruct]
+        if mps.to_right:
+            ddm = sum(m @ m.conj().T for m in mats) / len(mats)
+            U, S, qnn = _blocked_dm_eigh(ddm, qnbigl, qnbigr, mps.qntot)
+            m_trunc = mps.compress_config.compute_m_trunc(S, cidx[0], mps.to_right)
+        else:
+            ddm = sum(m.T @ m.conj() for m in mats) / len(mats)
+            U, S, qnn = _blocked_dm_eigh(ddm, qnbigr, qnbigl, mps.qntot)
+            m_trunc = mps.compress_config.compute_m_trunc(S, cidx[-1], mps.to_right)
+        sidx, msqn, _ = select_basis(None, S, qnn, None, m_trunc, percent=percent)
+        basis = _take_columns(U, sidx)
+        if mps.to_right:
+            ms = basis.reshape(lshape + [len(sidx)])
+            rotated = [(basis.conj().T @ m).reshape([len(sidx)] + rshape) for m in mats]
+        else:
+            ms = torch.movedim(basis.reshape(rshape + [len(sidx)]), -1, 0)
+            rotated = [(m @ basis.conj()).reshape(lshape + [len(sidx)]) for m in mats]
+        compms = rotated[0]
+    msqn = np.array(msqn).reshape(len(sidx), -1)
+    averaged = []
+    n = len(mps)
+
+    def td(a, b):
+        return torch.tensordot(a, b, dims=1)
+
+    if len(cidx) == 1:
+        c = cidx[0]
+        mps._mp[c] = ms.contiguous()
+        if mps.to_right:
+            if c != n - 1:
+                if multi:
+                    averaged = [td(r, mps._mp[c + 1]) for r in rotated]
+                mps._mp[c + 1] = td(compms, mps._mp[c + 1]).contiguous()
+                mps.qn[c + 1] = msqn
+                mps.qnidx = c + 1
+            else:
+                if multi:
+                    averaged = [td(mps._mp[c], r) for r in rotated]
+                mps._mp[c] = td(mps._mp[c], compms).contiguous()
+                mps.qnidx = n - 1
+        else:
+            if c != 0:
+                if multi:
+                    averaged = [td(mps._mp[c - 1], r) for r in rotated]
+                mps._mp[c - 1] = td(mps._mp[c - 1], compms).contiguous()
+                mps.qn[c] = msqn
+                mps.qnidx = c - 1
+            else:
+                if multi:
+                    averaged = [td(r, mps._mp[c]) for r in rotated]
+                mps._mp[c] = td(compms, mps._mp[c]).contiguous()
+                mps.qnidx = 0
+    else:
+        if mps.to_right:
+            mps._mp[cidx[0]] = ms.contiguous()
+            mps._mp[cidx[1]] = compms.contiguous()
+            mps.qnidx = cidx[1]
+        else:
+            mps._mp[cidx[1]] = ms.contiguous()
+            mps._mp[cidx[0]] = compms.contiguous()
+            mps.qnidx = cidx[0]
+        if multi:
+            averaged = rotated
+        mps.qn[cidx[1]] = msqn
+    mps._invalidate()
+    return averaged if multi else None
+
+
+# ---------------------------------------------------------------------------
+# sweeps
+
+
+def _single_sweep(mps: Mps, mpo: Mpo, wt, environ: Environ, percent, last_opt_e_idx):
+    cfg = mps.optimize_config
+    method, nroots, inverse = cfg.method, cfg.nroots, cfg.inverse
+    dev = mps._mp[0].device
+    n = len(mps)
+    averaged_ms = []
+    res_mps = None
+    micro = []
+    for imps in mps.iter_idx_list(full=True):
+        if method == "2site" and ((mps.to_right and imps == n - 1) or ((not mps.to_right) and imps == 0)):
+            break
+        lmethod, rmethod = ("System", "Enviro") if mps.to_right else ("Enviro", "System")
+        if method == "1site":
+            lidx, cidx, ridx = imps - 1, [imps], imps + 1
+        elif mps.to_right:
+            lidx, cidx, ridx = imps - 1, [imps, imps + 1], imps + 2
+        else:
+            lidx, cidx, ridx = imps - 2, [imps - 1, imps], imps + 1
+        ltensor = environ.GetLR("L", lidx, mps._mp, wt, itensor=None, method=lmethod)
+        rtensor = environ.GetLR("R", ridx, mps._mp, wt, itensor=None, method=rmethod)
+        qnbigl, qnbigr, qnmat = mps._get_big_qn(cidx)
+        qn_mask = get_qn_mask(qnmat, mps.qntot)
+        cshape = qn_mask.shape
+        W = _local_problem(mps, mpo, cidx, dev)
+        use_direct = np.prod(cshape) < 1000 or cfg.algo == "direct"
+        guesses = []
+        if not use_direct:
+            if nroots == 1:
+                raw = mps._mp[cidx[0]] if method == "1site" else \
+                    torch.tensordot(mps._mp[cidx[0]], mps._mp[cidx[1]], dims=1)
+                guesses = [raw.reshape(-1)]
+            else:
+                for ms in averaged_ms:
+                    if method == "1site":
+                        raw = ms
+                    elif mps.to_right:
+                        raw = torch.tensordot(ms, mps._mp[cidx[1]], dims=1)
+                    else:
+                        raw = torch.tensordot(mps._mp[cidx[0]], ms, dims=1)
+                    guesses.append(raw.reshape(-1))
+            nmask = int(qn_mask.sum())
+            for _ in range(len(guesses), nroots):
+                g = np.zeros(qn_mask.size)
+                g[qn_mask.reshape(-1)] = np.random.rand(nmask) - 0.5
+                guesses.append(torch.from_numpy(g).to(dev).to(torch.complex128))
+        e, cs, _ = _eigh_local(mps, ltensor, rtensor, W, qn_mask, guesses, nroots, inverse, use_direct)
+        if nroots > 1:
+            e = list(np.asarray(e).tolist())
+        micro.append((e, cidx))
+        if nroots == 1:
+            cstruct = cs[0].reshape(cshape)
+        else:
+            cstruct = [c.reshape(cshape) for c in cs]
+        if cidx == last_opt_e_idx:
+            if nroots == 1:
+                res_mps = mps.copy()
+                _update_mps(res_mps, cstruct, cidx, qnbigl, qnbigr, percent)
+            else:
+                res_mps = [mps.copy() for _ in cstruct]
+                for r, c in zip(res_mps, cstruct):
+                    _update_mps(r, c, cidx, qnbigl, qnbigr, percent)
+        averaged_ms = _update_mps(mps, cstruct, cidx, qnbigl, qnbigr, percent)
+    mps._switch_direction()
+    return micro, res_mps
+
+
+def optimize_mps(mps: Mps, mpo: Mpo, omega: float = None):
+    """DMRG (reference gs.py:54-171).  ``mps`` is overwritten during the
+    optimisation.  Returns (sweep energies, optimised MPS or list of MPS)."""
+    if omega is not None:
+        raise NotImplementedError("the omega target ((H-omega)^2, two-layer MPO) is not on the device path")
+    if not isinstance(mpo, Mpo):
+        raise NotImplementedError("StackedMpo is not on the device path")
+    cfg = mps.optimize_config
+    assert cfg.method in ("2site", "1site")
+    if cfg.algo not in ("davidson", "direct"):
+        raise NotImplementedError(f"eigensolver {cfg.algo} is not on the device path (davidson, direct)")
+    if mps.compress_config.ofs is not None:
+        raise NotImplementedError("on-the-fly site swapping is not on the device path")
+    if mps.on_device:
+        mps.to_host()
+    if mps.is_left_canonical:
+        mps.ensure_right_canonical()
+        env = "R"
+    else:
+        mps.ensure_left_canonical()
+        env = "L"
+    mps.to_complex(inplace=True)
+    mps.to_device()
+    dev = mps._mp[0].device
+    wt = mpo.device_tensors(native.PREC_FP64, dev)
+    compress_config_bk = mps.compress_config
+    environ = Environ(mps._mp, wt, env)
+    macro = []
+    opt_e_idx = None
+    res_mps: Union[Mps, List[Mps]] = None
+    for isweep, (compress_config, percent) in enumerate(cfg.procedure):
+        if isinstance(compress_config, CompressConfig):
+            mps.compress_config = compress_config
+        elif isinstance(compress_config, (int, np.integer)):
+            mps.compress_config = CompressConfig(criteria=CompressCriteria.fixed,
+                                                 max_bonddim=int(compress_config))
+        else:
+            raise AssertionError(f"procedure entry {compress_config}")
+        micro, sweep_res = _single_sweep(mps, mpo, wt, environ, percent, opt_e_idx)
+        if sweep_res is not None:
+            res_mps = sweep_res
+        opt_e = min(micro, key=lambda t: (t[0], t[1]))
+        macro.append(opt_e[0])
+        opt_e_idx = opt_e[1]
+        logger.debug("sweep %d: lowest energy %s", isweep + 1, min(macro))
+        if isweep > 0 and percent == 0:
+            v1, v2 = sorted(macro)[:2]
+            if np.allclose(v1, v2, rtol=cfg.e_rtol, atol=cfg.e_atol):
+                logger.info("DMRG has converged")
+                break
+    else:
+        logger.warning("DMRG did not converge; lowest two energies %s", sorted(macro)[:2])
+    assert res_mps is not None
+
+    def finish(m):
+        m.to_host()
+        m.normalize("mps_only")
+        m.ensure_left_canonical()
+        m.canonicalise()
+        m.compress_config = compress_config_bk
+        return m
+
+    if cfg.nroots == 1:
+        return macro, finish(res_mps)
+    return macro, [finish(m) for m in res_mps]

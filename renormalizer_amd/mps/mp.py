@@ -270,6 +270,49 @@ class MatrixProduct:
             e0 = np.tensordot(e0, mt1, axes).T
         return complex(e0[0, 0])
 
+    # ---- canonical form (reference mp.py:174-228) --------------------------
+    @property
+    def is_left_canonical(self):
+        return self.qnidx == self.site_num - 1
+
+    @property
+    def is_right_canonical(self):
+        return self.qnidx == 0
+
+    def _site_host(self, i):
+        m = self[i]
+        return m.cpu().numpy() if _is_dev(m) else np.asarray(m)
+
+    def check_left_canonical(self, rtol=1e-5, atol=1e-8):
+        for i in range(len(self) - 1):
+            m = self._site_host(i)
+            m = m.reshape(-1, m.shape[-1])
+            if not np.allclose(m.conj().T @ m, np.eye(m.shape[1]), rtol=rtol, atol=atol):
+                return False
+        return True
+
+    def check_right_canonical(self, rtol=1e-5, atol=1e-8):
+        for i in range(1, len(self)):
+            m = self._site_host(i)
+            m = m.reshape(m.shape[0], -1)
+            if not np.allclose(m @ m.conj().T, np.eye(m.shape[0]), rtol=rtol, atol=atol):
+                return False
+        return True
+
+    def ensure_left_canonical(self, rtol=1e-5, atol=1e-8):
+        if self.to_right or self.qnidx != self.site_num - 1 or not self.check_left_canonical(rtol, atol):
+            self.move_qnidx(0)
+            self.to_right = True
+            return self.canonicalise()
+        return self
+
+    def ensure_right_canonical(self, rtol=1e-5, atol=1e-8):
+        if (not self.to_right) or self.qnidx != 0 or not self.check_right_canonical(rtol, atol):
+            self.move_qnidx(self.site_num - 1)
+            self.to_right = False
+            return self.canonicalise()
+        return self
+
     def _push_cano(self, idx):
         mt = self[idx]
         assert np.any(mt)

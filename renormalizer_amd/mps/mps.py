@@ -23,6 +23,7 @@ from typing import Dict, List, Union
 
 import numpy as np
 import torch
+import scipy.linalg
 from scipy import stats
 
 from renormalizer_amd import native
@@ -31,7 +32,7 @@ from renormalizer_amd.model import basis as ba
 from renormalizer_amd.mps.lib import Environ
 from renormalizer_amd.mps.mp import MatrixProduct, _is_dev
 from renormalizer_amd.mps.mpo import Mpo
-from renormalizer_amd.mps.svd_qn import svd_qn_device
+from renormalizer_amd.mps.svd_qn import add_outer, get_qn_mask, select_basis, svd_qn_device
 from renormalizer_amd.utils import (CompressConfig, EvolveConfig, EvolveMethod, OptimizeConfig)
 
 logger = logging.getLogger(__name__)
@@ -153,6 +154,59 @@ class Mps(MatrixProduct):
             else:
                 raise NotImplementedError(type(local_basis))
             mps._mp[isite] = ms
+        return mps
+
+    @classmethod
+    def random(cls, model: Model, qntot, m_max, percent=1.0) -> "Mps":
+        """Random left-canonical MPS in the sector ``qntot`` (reference
+        mps.py:120-185): on every bond, random orthonormal vectors inside each
+        allowed quantum-number sector (eigenvectors of random symmetric
+        blocks) with ``select_basis`` choosing the kept ones; the last site is
+        random and normalised.  Host tensors, numpy's global generator."""
+        mps = cls()
+        mps.model = model
+        if isinstance(qntot, (int, np.integer)):
+            qntot = np.array([qntot])
+        qntot = np.asarray(qntot)
+        qn_size = len(qntot)
+        assert qn_size == model.qn_size
+        mps.qn = [np.zeros((1, qn_size), dtype=int)]
+        mps._mp = []
+        dim_list = [1]
+        for imps in range(model.nsite - 1):
+            qnbig = add_outer(mps.qn[imps], np.array(model.basis[imps].sigmaqn)).reshape(-1, qn_size)
+            u_set, s_set, qnset = [], [], []
+            for iblock in set([tuple(t) for t in qnbig]):
+                if np.all(qntot < np.array(iblock)):
+                    continue
+                indices = [i for i, x in enumerate(qnbig) if tuple(x) == iblock]
+                a = np.random.random([len(indices), len(indices)]) - 0.5
+                a = a + a.T
+                s, u = scipy.linalg.eigh(a=a)
+                rec = np.zeros((len(qnbig), len(indices)))
+                rec[indices, :] = u
+                u_set.append(rec)
+                s_set.append(s)
+                qnset += [iblock] * len(indices)
+            u_set = np.concatenate(u_set, axis=1)
+            s_set = np.concatenate(s_set)
+            m_max2 = m_max[imps + 1] if isinstance(m_max, (list, tuple, np.ndarray)) else m_max
+            sidx, mpsqn, _ = select_basis(u_set, s_set, qnset, None, m_max2, percent=percent)
+            mt = u_set[:, sidx]
+            dim_list.append(len(sidx))
+            mps._mp.append(mt.reshape((dim_list[imps], -1, dim_list[imps + 1])))
+            mps.qn.append(mpsqn.reshape(len(sidx), qn_size))
+        mps.qn.append(np.zeros((1, qn_size), dtype=int))
+        dim_list.append(1)
+        last_mt = np.random.random([dim_list[-2], model.basis[-1].nbas, dim_list[-1]]) - 0.5
+        qnmask = get_qn_mask(add_outer(mps.qn[-2], np.array(model.basis[-1].sigmaqn)), qntot)
+        last_mt[~qnmask] = 0
+        last_mt /= np.linalg.norm(last_mt.ravel())
+        mps._mp.append(last_mt)
+        mps.qnidx = len(mps._mp) - 1
+        mps.to_right = False
+        mps.qntot = qntot
+        mps.dtype = np.float64
         return mps
 
     # ---- bookkeeping -------------------------------------------------------

@@ -121,3 +121,34 @@ def svd_qn_host(coef_array, qnbigl, qnbigr, qntot, QR=False, system=None, full_m
         new_qnl = np.array(new_qnl)[order].tolist()
         new_qnr = np.array(new_qnr)[order].tolist()
     return u, su, new_qnl, v, sv, new_qnr
+
+
+def select_basis(vset, sset, qnlist, compset, m_max, percent=0):
+    """Renormalised basis selection (reference mps/lib.py:253-322): ``percent``
+    of the kept states is shared equally among the quantum-number sectors
+    (largest weights first inside each), the rest goes to the largest weights
+    overall.  Host bookkeeping only: returns the chosen column indices ``sidx``
+    (into vset / compset), the kept quantum numbers and the kept weights."""
+    qnlist = [tuple(qn) for qn in qnlist]
+    qnset = set(qnlist)
+    basdic = {i: [qnlist[i], sset[i]] for i in range(len(qnlist)) if qnlist[i] in qnset}
+
+    def block_select(qn, n):
+        block = sorted(((i, v) for i, v in basdic.items() if v[0] == qn), key=lambda x: x[1][1],
+                       reverse=True)
+        sel = [i for i, _ in block[:min(n, len(block))]]
+        for i in sel:
+            del basdic[i]
+        return sel
+
+    nbasis = min(len(basdic), m_max)
+    sidx = []
+    if percent != 0:
+        nbas_block = int(nbasis * percent / len(qnset))
+        for iqn in qnset:
+            sidx += block_select(iqn, nbas_block)
+    nbasis = nbasis - len(sidx)
+    rest = sorted(basdic.items(), key=lambda x: x[1][1], reverse=True)
+    sidx += [i for i, _ in rest[:nbasis]]
+    assert len(sidx) == len(set(sidx))
+    return sidx, np.array([qnlist[i] for i in sidx]), np.array([sset[i] for i in sidx])

@@ -32,7 +32,7 @@ EXPORTS = (
     "reno_hop_workspace_bytes", "reno_krylov_workspace_bytes",
     "reno_env_workspace_bytes", "reno_qr_workspace_bytes",
     "reno_hop_apply", "reno_env_update", "reno_env_close", "reno_expm_krylov",
-    "reno_expm_krylov_async", "reno_block_qr", "reno_zgemm", "reno_probe_mfma16", "reno_prof_enable",
+    "reno_expm_krylov_async", "reno_block_qr", "reno_heff_diag", "reno_heff_dense", "reno_zgemm", "reno_probe_mfma16", "reno_prof_enable",
     "reno_prof_collect",
 )
 
@@ -100,6 +100,10 @@ def load_library(path: str = LIB_PATH):
         lib.reno_expm_krylov_async.restype = c_int
         lib.reno_expm_krylov_async.argtypes = [ctypes.POINTER(HopDesc), c_double, c_double, vp, vp,
                                                c_int, vp, vp, c_size, vp]
+        lib.reno_heff_diag.restype = c_int
+        lib.reno_heff_diag.argtypes = [c_int] * 5 + [vp, vp, vp, vp, vp]
+        lib.reno_heff_dense.restype = c_int
+        lib.reno_heff_dense.argtypes = [c_int] * 5 + [vp, vp, vp, vp, c_int, vp, vp]
         lib.reno_block_qr.restype = c_int
         lib.reno_block_qr.argtypes = [c_int, c_int, c_int, vp, c_int, ctypes.POINTER(QnBlock),
                                       vp, vp, c_int, vp, c_size, vp]
@@ -327,6 +331,44 @@ def expm_krylov_async(L, W, R, v, dt, prec, max_matvec, info):
     check(l.reno_expm_krylov_async(ctypes.byref(desc), dt.real, dt.imag, _ptr(v), _ptr(out),
                                    max_matvec, _ptr(info), _ptr(ws), ws.numel(), _stream()),
           "reno_expm_krylov_async")
+    return out
+
+
+def _heff_args(L, W, R, shape):
+    _require_device(L, W, R)
+    Ml, d, Mr = shape
+    Dl, Dr = W.shape[0], W.shape[3]
+    if L.dtype != torch.complex128 or R.dtype != torch.complex128 or W.dtype != torch.float64:
+        raise ValueError("H_eff helpers run in FP64: complex128 environments, float64 MPO")
+    assert L.shape == (Ml, Dl, Ml) and R.shape == (Mr, Dr, Mr) and W.shape == (Dl, d, d, Dr), \
+        "H_eff shape mismatch"
+    for t in (L, W, R):
+        if not t.is_contiguous():
+            raise ValueError("H_eff operands must be contiguous")
+    return Ml, d, Mr, Dl, Dr
+
+
+def heff_diag(L, W, R, shape):
+    """Diagonal of the one-site effective Hamiltonian, shape (Ml, d, Mr)
+    (reference gs.py:424-436)."""
+    Ml, d, Mr, Dl, Dr = _heff_args(L, W, R, shape)
+    out = torch.empty((Ml, d, Mr), dtype=torch.complex128, device=L.device)
+    check(lib().reno_heff_diag(Ml, d, Mr, Dl, Dr, _ptr(L), _ptr(W), _ptr(R), _ptr(out), _stream()),
+          "reno_heff_diag")
+    return out
+
+
+def heff_dense(L, W, R, shape, idx):
+    """Dense effective Hamiltonian on the flat site indices ``idx`` (device
+    int64), shape (len(idx), len(idx)) (reference gs.py:302-330)."""
+    Ml, d, Mr, Dl, Dr = _heff_args(L, W, R, shape)
+    _require_device(idx)
+    if idx.dtype != torch.int64 or not idx.is_contiguous():
+        raise ValueError("idx must be a contiguous int64 device tensor")
+    n = idx.numel()
+    out = torch.empty((n, n), dtype=torch.complex128, device=L.device)
+    check(lib().reno_heff_dense(Ml, d, Mr, Dl, Dr, _ptr(L), _ptr(W), _ptr(R), _ptr(idx), n, _ptr(out),
+                                _stream()), "reno_heff_dense")
     return out
 
 
