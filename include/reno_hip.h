@@ -172,11 +172,14 @@ int reno_zgemm(int M, int N, int K, const void* A, const void* B, void* C,
  * the benchmark's roofline figure).  reno_prof_enable(1) clears the record
  * and brackets every subsequent GEMM launch with HIP events on its stream;
  * reno_prof_enable(0) stops recording.  reno_prof_collect waits for the
- * events and fills, per kernel instantiation k (16 kinds:
- * k = 8*(prec==MIXED) + 4*A_k_contiguous + 2*B_k_contiguous + A_real),
+ * events and fills, per kernel instantiation k (see reno_prof_kind_info),
  * the launch count, summed GPU milliseconds and summed algorithmic FLOP. */
 int reno_prof_enable(int on);
 int reno_prof_collect(int nkinds, double* ms, double* flop, int* count);
+/* Number of GEMM kernel instantiations; for 0 <= kind < that number, fills
+ * info[0..5] = precision, A k-contiguous, B k-contiguous, and the storage
+ * kinds of A, B, C (0 complex128, 1 complex64, 2 float64, 3 float32). */
+int reno_prof_kind_info(int kind, int* info);
 
 /* ---- test hook: one v_mfma 16x16x4 tile, used by the lane-layout check.
  * a (16x4), b (4x16), c (16x16) row-major; prec selects f64 or f32. */

@@ -45,54 +45,44 @@ template <> struct Mfma<float> {
   __device__ static __forceinline__ int out_row(int q, int r) { return 4 * q + r; }
 };
 
-template <typename R>
-__device__ __forceinline__ void load_elem(const void* base, int kind, int64_t off,
-                                          R& re, R& im) {
-  switch (kind) {
-    case ST_Z128: {
-      double2 v = reinterpret_cast<const double2*>(base)[off];
-      re = (R)v.x;
-      im = (R)v.y;
-      break;
-    }
-    case ST_C64: {
-      float2 v = reinterpret_cast<const float2*>(base)[off];
-      re = (R)v.x;
-      im = (R)v.y;
-      break;
-    }
-    case ST_F64:
-      re = (R)reinterpret_cast<const double*>(base)[off];
-      im = (R)0;
-      break;
-    default:
-      re = (R)reinterpret_cast<const float*>(base)[off];
-      im = (R)0;
-      break;
+// Element access with the storage kind fixed at compile time (a runtime
+// switch per element would split every load into branches and serialise the
+// memory latency).
+template <typename R, int KIND>
+__device__ __forceinline__ void load_k(const void* base, int64_t off, R& re, R& im) {
+  if constexpr (KIND == ST_Z128) {
+    const double2 v = reinterpret_cast<const double2*>(base)[off];
+    re = (R)v.x;
+    im = (R)v.y;
+  } else if constexpr (KIND == ST_C64) {
+    const float2 v = reinterpret_cast<const float2*>(base)[off];
+    re = (R)v.x;
+    im = (R)v.y;
+  } else if constexpr (KIND == ST_F64) {
+    re = (R)reinterpret_cast<const double*>(base)[off];
+    im = (R)0;
+  } else {
+    re = (R)reinterpret_cast<const float*>(base)[off];
+    im = (R)0;
   }
 }
 
-template <typename R>
-__device__ __forceinline__ void store_elem(void* base, int kind, int64_t off, R re,
-                                           R im, int beta) {
-  if (kind == ST_Z128) {
+template <typename R, int KIND>
+__device__ __forceinline__ void store_k(void* base, int64_t off, R re, R im, int beta) {
+  if constexpr (KIND == ST_Z128) {
     double2* p = reinterpret_cast<double2*>(base) + off;
-    double2 v;
-    v.x = (double)re;
-    v.y = (double)im;
+    double2 v = make_double2((double)re, (double)im);
     if (beta) {
-      double2 o = *p;
+      const double2 o = *p;
       v.x += o.x;
       v.y += o.y;
     }
     *p = v;
   } else {
     float2* p = reinterpret_cast<float2*>(base) + off;
-    float2 v;
-    v.x = (float)re;
-    v.y = (float)im;
+    float2 v = make_float2((float)re, (float)im);
     if (beta) {
-      float2 o = *p;
+      const float2 o = *p;
       v.x += o.x;
       v.y += o.y;
     }
@@ -100,20 +90,29 @@ __device__ __forceinline__ void store_elem(void* base, int kind, int64_t off, R 
   }
 }
 
+// runtime-kind store (split-K reduction epilogue)
+template <typename R>
+__device__ __forceinline__ void store_elem(void* base, int kind, int64_t off, R re, R im, int beta) {
+  if (kind == ST_Z128) store_k<R, ST_Z128>(base, off, re, im, beta);
+  else store_k<R, ST_C64>(base, off, re, im, beta);
+}
+
 constexpr int kBM = 64;
 constexpr int kBN = 64;
 constexpr int kBK = 16;
 constexpr int kThreads = 256;
-constexpr int kPad = 4;
+constexpr int kPad = 1;  // odd row stride: conflict-free LDS writes along k
 constexpr int kPerThread = (kBM * kBK) / kThreads;  // 4 (A) and 4 (B)
 
 // AK: A is contiguous along k (sAk == 1), else along m.
 // BKC: B is contiguous along k (sBk == 1), else along n.
 // AREAL: A has no imaginary part.
-template <typename R, bool AK, bool BKC, bool AREAL>
+template <typename R, bool AK, bool BKC, int AKIND, int BKIND, int CKIND>
 __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
-  __shared__ R sA[2][kBK][kBM + kPad];
-  __shared__ R sB[2][kBK][kBN + kPad];
+  constexpr bool AREAL = (AKIND == ST_F64 || AKIND == ST_F32);
+  // double-buffered: [buffer][re / im plane][k][m or n]
+  __shared__ R sA[2][2][kBK][kBM + kPad];
+  __shared__ R sB[2][2][kBK][kBN + kPad];
   typedef typename Mfma<R>::acc acc_t;
   if (g.skip && *g.skip) return;
   if (g.exec && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0)
@@ -186,7 +185,7 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
       const int gm = m0 + mm, gk = k0 + kk;
       R re = (R)0, im = (R)0;
       if (gm < g.M && gk < ke)
-        load_elem<R>(g.A, g.a_kind, aoff + (int64_t)gm * g.sAm + (int64_t)gk * g.sAk, re, im);
+        load_k<R, AKIND>(g.A, aoff + (int64_t)gm * g.sAm + (int64_t)gk * g.sAk, re, im);
       pa_re[i] = re;
       pa_im[i] = g.conjA ? -im : im;
       int nn;
@@ -196,49 +195,53 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
       re = (R)0;
       im = (R)0;
       if (gn < g.N && gk2 < ke)
-        load_elem<R>(g.B, g.b_kind, boff + (int64_t)gk2 * g.sBk + (int64_t)gn * g.sBn, re, im);
+        load_k<R, BKIND>(g.B, boff + (int64_t)gk2 * g.sBk + (int64_t)gn * g.sBn, re, im);
       pb_re[i] = re;
       pb_im[i] = g.conjB ? -im : im;
     }
   };
 
-  auto sstore = [&]() {
+  auto sstore = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i) {
       const int e = tid + i * kThreads;
       int mm, kk;
       a_coord(e, mm, kk);
-      sA[0][kk][mm] = pa_re[i];
-      if (!AREAL) sA[1][kk][mm] = pa_im[i];
+      sA[buf][0][kk][mm] = pa_re[i];
+      if (!AREAL) sA[buf][1][kk][mm] = pa_im[i];
       int nn;
       b_coord(e, kk, nn);
-      sB[0][kk][nn] = pb_re[i];
-      sB[1][kk][nn] = pb_im[i];
+      sB[buf][0][kk][nn] = pb_re[i];
+      sB[buf][1][kk][nn] = pb_im[i];
     }
   };
 
   const int r16 = lane & 15;
   const int kq = lane >> 4;
 
+  // one barrier per K step: the next tile goes to the other LDS buffer while
+  // this one is multiplied (all waves left the other buffer at the previous
+  // barrier)
   gload(kb);
+  sstore(0);
+  __syncthreads();
+  int cur = 0;
   for (int k0 = kb; k0 < ke; k0 += kBK) {
-    __syncthreads();
-    sstore();
-    __syncthreads();
-    if (k0 + kBK < ke) gload(k0 + kBK);
+    const bool more = k0 + kBK < ke;
+    if (more) gload(k0 + kBK);
 #pragma unroll
     for (int ks = 0; ks < kBK / 4; ++ks) {
       const int kr = ks * 4 + kq;
       R a_re[2], a_im[2], b_re[2], b_im[2];
 #pragma unroll
       for (int mb = 0; mb < 2; ++mb) {
-        a_re[mb] = sA[0][kr][wm * 32 + mb * 16 + r16];
-        a_im[mb] = AREAL ? (R)0 : sA[1][kr][wm * 32 + mb * 16 + r16];
+        a_re[mb] = sA[cur][0][kr][wm * 32 + mb * 16 + r16];
+        a_im[mb] = AREAL ? (R)0 : sA[cur][1][kr][wm * 32 + mb * 16 + r16];
       }
 #pragma unroll
       for (int nb = 0; nb < 2; ++nb) {
-        b_re[nb] = sB[0][kr][wn * 32 + nb * 16 + r16];
-        b_im[nb] = sB[1][kr][wn * 32 + nb * 16 + r16];
+        b_re[nb] = sB[cur][0][kr][wn * 32 + nb * 16 + r16];
+        b_im[nb] = sB[cur][1][kr][wn * 32 + nb * 16 + r16];
       }
 #pragma unroll
       for (int mb = 0; mb < 2; ++mb)
@@ -252,6 +255,9 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
           }
         }
     }
+    if (more) sstore(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
   }
 
 #pragma unroll
@@ -263,29 +269,54 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
         const int row = m0 + wm * 32 + mb * 16 + Mfma<R>::out_row(kq, r);
         const int col = n0 + wn * 32 + nb * 16 + r16;
         if (row < g.M && col < g.N)
-          store_elem<R>(g.C, g.c_kind, coff + (int64_t)row * g.sCm + (int64_t)col * g.sCn,
-                        cr[mb][nb][r], ci[mb][nb][r], g.beta);
+          store_k<R, CKIND>(g.C, coff + (int64_t)row * g.sCm + (int64_t)col * g.sCn,
+                            cr[mb][nb][r], ci[mb][nb][r], g.beta);
       }
 }
 
-template <typename R, bool AK, bool BKC>
-static void launch_t(const GemmCall& g, dim3 grid, hipStream_t s, bool areal) {
-  if (areal)
-    hipLaunchKernelGGL((k_cgemm<R, AK, BKC, true>), grid, dim3(kThreads), 0, s, g);
-  else
-    hipLaunchKernelGGL((k_cgemm<R, AK, BKC, false>), grid, dim3(kThreads), 0, s, g);
-}
+// ---- instantiations -----------------------------------------------------
+// The (precision, A k-contiguous, B k-contiguous, A/B/C storage) combinations
+// the contractions use; launch_gemm rejects any other.  Their order is the
+// "kind" index of the timing hook (reno_prof_kind_info).
+typedef void (*GemmKernel)(GemmCall);
+struct GemmKind {
+  int prec, ak, bk, akind, bkind, ckind;
+  GemmKernel fn;
+};
+#define RENO_KIND(R, P, AK, BK, A, B, C) \
+  {P, AK, BK, A, B, C, k_cgemm<R, (bool)AK, (bool)BK, A, B, C>}
+static const GemmKind kGemmKinds[] = {
+    // FP64: complex128 operands, real MPO stage
+    RENO_KIND(double, RENO_PREC_FP64, 1, 0, ST_Z128, ST_Z128, ST_Z128),
+    RENO_KIND(double, RENO_PREC_FP64, 1, 1, ST_Z128, ST_Z128, ST_Z128),
+    RENO_KIND(double, RENO_PREC_FP64, 0, 0, ST_Z128, ST_Z128, ST_Z128),
+    RENO_KIND(double, RENO_PREC_FP64, 0, 1, ST_Z128, ST_Z128, ST_Z128),
+    RENO_KIND(double, RENO_PREC_FP64, 1, 0, ST_F64, ST_Z128, ST_Z128),
+    // mixed: complex64 environments / intermediates, complex128 site vectors
+    RENO_KIND(float, RENO_PREC_MIXED, 1, 0, ST_C64, ST_Z128, ST_C64),   // S1
+    RENO_KIND(float, RENO_PREC_MIXED, 1, 0, ST_F32, ST_C64, ST_C64),    // S2
+    RENO_KIND(float, RENO_PREC_MIXED, 1, 1, ST_C64, ST_C64, ST_Z128),   // S3 -> vector
+    RENO_KIND(float, RENO_PREC_MIXED, 1, 1, ST_C64, ST_C64, ST_C64),    // S3 split-K partials
+    RENO_KIND(float, RENO_PREC_MIXED, 0, 0, ST_Z128, ST_C64, ST_C64),   // env L S3
+    RENO_KIND(float, RENO_PREC_MIXED, 1, 1, ST_Z128, ST_C64, ST_C64),   // env R S1 / S3
+};
+#undef RENO_KIND
+static constexpr int kNumGemmKinds = (int)(sizeof(kGemmKinds) / sizeof(kGemmKinds[0]));
 
-template <typename R>
-static void launch_r(const GemmCall& g, dim3 grid, hipStream_t s, bool ak, bool bk,
-                     bool areal) {
-  if (ak) {
-    if (bk) launch_t<R, true, true>(g, grid, s, areal);
-    else launch_t<R, true, false>(g, grid, s, areal);
-  } else {
-    if (bk) launch_t<R, false, true>(g, grid, s, areal);
-    else launch_t<R, false, false>(g, grid, s, areal);
+// The kernels address A and B through both strides, so the k-contiguity
+// flags only choose the coalescing pattern: without an exact match (a
+// degenerate dimension of size 1 makes both patterns valid) any
+// instantiation with the same precision and storage kinds is correct.
+static int find_kind(const GemmCall& g, bool ak, bool bk) {
+  int any = -1;
+  for (int i = 0; i < kNumGemmKinds; ++i) {
+    const GemmKind& k = kGemmKinds[i];
+    if (k.prec != g.prec || k.akind != g.a_kind || k.bkind != g.b_kind || k.ckind != g.c_kind)
+      continue;
+    if (k.ak == (int)ak && k.bk == (int)bk) return i;
+    if (any < 0) any = i;
   }
+  return any;
 }
 
 // ---- split-K -------------------------------------------------------------
@@ -348,10 +379,6 @@ static std::vector<hipEvent_t> g_event_pool;
 static constexpr size_t kExecSlots = 1 << 20;
 static int* g_exec = nullptr;  // kExecSlots markers, allocated on first enable
 
-static int kind_index(int prec, bool ak, bool bk, bool areal) {
-  return (prec == RENO_PREC_FP64 ? 0 : 8) + (ak ? 4 : 0) + (bk ? 2 : 0) + (areal ? 1 : 0);
-}
-
 static hipEvent_t pooled_event() {
   if (!g_event_pool.empty()) {
     hipEvent_t e = g_event_pool.back();
@@ -405,6 +432,7 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
   }
   GemmCall k = g;
   k.kchunk = kchunk;
+  int kid = -1;
   if (nsplit > 1) {
     // partial sums [split][batch][M][N] in the compute precision, overwritten
     k.C = g.part;
@@ -413,6 +441,18 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
     k.sCn = 1;
     k.sCb = (int64_t)g.M * g.N;
     k.beta = 0;
+    kid = find_kind(k, ak, bk);
+    if (kid < 0) {  // no instantiation for the partials: run unsplit
+      nsplit = 1;
+      k = g;
+      k.kchunk = g.K > 0 ? g.K : 1;
+    }
+  }
+  if (kid < 0) kid = find_kind(k, ak, bk);
+  if (kid < 0) {
+    set_error("gemm: no kernel for prec=%d A k-contiguous=%d B k-contiguous=%d kinds A=%d B=%d C=%d",
+              g.prec, (int)ak, (int)bk, g.a_kind, g.b_kind, k.c_kind);
+    return RENO_EINVAL;
   }
   dim3 grid((unsigned)tiles, (unsigned)g.batch, (unsigned)nsplit);
   // K == 0 needs no special case: the kernel stores its zero accumulators.
@@ -430,10 +470,11 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
     }
   }
   if (prof) (void)hipEventRecord(rec.a, stream);
-  if (g.prec == RENO_PREC_FP64)
-    launch_r<double>(k, grid, stream, ak, bk, areal);
-  else
-    launch_r<float>(k, grid, stream, ak, bk, areal);
+  {
+    void* args[] = {(void*)&k};
+    RENO_HIP_CHECK(hipLaunchKernel((const void*)kGemmKinds[kid].fn, grid, dim3(kThreads), args, 0,
+                                   stream));
+  }
   RENO_CHECK_LAUNCH();
   // the timing bracket covers the matrix-core kernel only (the split-K
   // reduction is a separate, bandwidth-bound kernel)
@@ -453,7 +494,7 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
   }
   if (prof) {
     rec.flop = (areal ? 4.0 : 8.0) * (double)g.M * (double)g.N * (double)g.K * (double)g.batch;
-    rec.kind = kind_index(g.prec, ak, bk, areal);
+    rec.kind = kid;
     std::lock_guard<std::mutex> lk(g_prof_mu);
     rec.slot = k.exec ? (int)(k.exec - g_exec) : -1;
     g_prof.push_back(rec);
@@ -543,6 +584,19 @@ extern "C" int reno_probe_mfma16(int prec, const void* a, const void* b, void* c
 }
 
 extern "C" int reno_prof_enable(int on) { return reno::prof_enable(on); }
+
+extern "C" int reno_prof_kind_info(int kind, int* info) {
+  if (kind >= 0 && kind < reno::kNumGemmKinds && info) {
+    const reno::GemmKind& k = reno::kGemmKinds[kind];
+    info[0] = k.prec;
+    info[1] = k.ak;
+    info[2] = k.bk;
+    info[3] = k.akind;
+    info[4] = k.bkind;
+    info[5] = k.ckind;
+  }
+  return reno::kNumGemmKinds;
+}
 
 extern "C" int reno_prof_collect(int nkinds, double* ms, double* flop, int* count) {
   return reno::prof_collect(nkinds, ms, flop, count);

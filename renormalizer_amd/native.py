@@ -38,12 +38,19 @@ EXPORTS = (
 
 # demangled names of the GEMM kernel instantiations, indexed like
 # reno_prof_collect's kinds
-GEMM_KINDS = [
-    "void reno::k_cgemm<%s, %s, %s, %s>(reno::GemmCall)" % (
-        "double" if k < 8 else "float", "true" if k & 4 else "false",
-        "true" if k & 2 else "false", "true" if k & 1 else "false")
-    for k in range(16)
-]
+def gemm_kind_names():
+    """Names (as profilers print them) of the GEMM kernel instantiations, in
+    the order of reno_prof_collect's kinds."""
+    l = lib()
+    n = l.reno_prof_kind_info(-1, None)
+    names = []
+    info = (ctypes.c_int * 6)()
+    for k in range(n):
+        l.reno_prof_kind_info(k, info)
+        names.append("void reno::k_cgemm<%s, %s, %s, %d, %d, %d>(reno::GemmCall)" % (
+            "double" if info[0] == PREC_FP64 else "float", "true" if info[1] else "false",
+            "true" if info[2] else "false", info[3], info[4], info[5]))
+    return names
 
 
 class RenoNativeError(RuntimeError):
@@ -115,6 +122,8 @@ def load_library(path: str = LIB_PATH):
         lib.reno_probe_mfma16.argtypes = [c_int, vp, vp, vp, vp]
         lib.reno_prof_enable.restype = c_int
         lib.reno_prof_enable.argtypes = [c_int]
+        lib.reno_prof_kind_info.restype = c_int
+        lib.reno_prof_kind_info.argtypes = [c_int, ctypes.POINTER(c_int)]
         lib.reno_prof_collect.restype = c_int
         lib.reno_prof_collect.argtypes = [c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_double),
                                           ctypes.POINTER(c_int)]
@@ -460,12 +469,13 @@ def prof_enable(on: bool):
 
 def prof_collect():
     """{kernel name: (launches, total ms, total algorithmic FLOP)}."""
-    n = len(GEMM_KINDS)
+    names = gemm_kind_names()
+    n = len(names)
     ms = (ctypes.c_double * n)()
     fl = (ctypes.c_double * n)()
     ct = (ctypes.c_int * n)()
     check(lib().reno_prof_collect(n, ms, fl, ct), "reno_prof_collect")
-    return {GEMM_KINDS[k]: (ct[k], ms[k], fl[k]) for k in range(n) if ct[k]}
+    return {names[k]: (ct[k], ms[k], fl[k]) for k in range(n) if ct[k]}
 
 
 def probe_mfma16(a, b, prec):
