@@ -103,7 +103,7 @@ HopLayout hop_layout(const reno_hop_desc* op) {
 // out = H_eff * cin with pre-permuted Wl (nsite 1) and scratch t1/t2.
 int hop_apply_core(const reno_hop_desc* op, const void* Wl, const void* cin, void* cout,
                    void* t1, void* t2, hipStream_t s, const int* skip, void* part,
-                   size_t part_bytes) {
+                   size_t part_bytes, const double2* dot_v, double* dot_part, int dot_np) {
   const int prec = op->prec;
   const int ck = cplx_kind(prec);
   const int64_t Ml = op->Ml, d = op->d, Mr = op->Mr, Dl = op->Dl, Dr = op->Dr;
@@ -137,6 +137,9 @@ int hop_apply_core(const reno_hop_desc* op, const void* Wl, const void* cin, voi
     g3.skip = skip;
     g3.part = part;
     g3.part_bytes = part_bytes;
+    g3.dot_v = dot_v;
+    g3.dot_part = dot_part;
+    g3.dot_np = dot_np;
     RENO_TRY(launch_gemm(g3, s));
   } else {
     const int64_t D = Dl;
@@ -159,6 +162,9 @@ int hop_apply_core(const reno_hop_desc* op, const void* Wl, const void* cin, voi
     g2.skip = skip;
     g2.part = part;
     g2.part_bytes = part_bytes;
+    g2.dot_v = dot_v;
+    g2.dot_part = dot_part;
+    g2.dot_np = dot_np;
     RENO_TRY(launch_gemm(g2, s));
   }
   return RENO_OK;

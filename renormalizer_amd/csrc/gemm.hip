@@ -361,6 +361,66 @@ __global__ void k_split_reduce(const typename std::conditional<sizeof(R) == 8, d
   }
 }
 
+// 256-thread fixed-order sum (deterministic)
+__device__ __forceinline__ double block_sum256(double v, double* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (tid < off) sh[tid] += sh[tid + off];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// Split-K reduction of a contiguous complex128 result fused with the dot
+// product the Lanczos recurrence needs next: C = sum_s P[s] (+ C), and
+// dot_part[block] = sum over the block's share of Re(conj(C) v).
+template <typename R>
+__global__ __launch_bounds__(256) void k_split_reduce_dot(
+    const typename std::conditional<sizeof(R) == 8, double2, float2>::type* __restrict__ P,
+    int nsplit, int64_t total, double2* __restrict__ C, int beta, const double2* __restrict__ v,
+    double* __restrict__ dot_part, const int* skip) {
+  if (skip && *skip) return;
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * 256) {
+    R re = (R)0, im = (R)0;
+    for (int s = 0; s < nsplit; ++s) {
+      const auto x = P[(int64_t)s * total + t];
+      re += x.x;
+      im += x.y;
+    }
+    double2 c = make_double2((double)re, (double)im);
+    if (beta) {
+      const double2 o = C[t];
+      c.x += o.x;
+      c.y += o.y;
+    }
+    C[t] = c;
+    const double2 w = v[t];
+    acc += c.x * w.x + c.y * w.y;
+  }
+  const double sum = block_sum256(acc, sh);
+  if (threadIdx.x == 0) dot_part[blockIdx.x] = sum;
+}
+
+// dot_part[block] = partial sums of Re(conj(c) v) (unsplit GEMM results)
+__global__ __launch_bounds__(256) void k_dot_part(const double2* __restrict__ c,
+                                                  const double2* __restrict__ v, int64_t n,
+                                                  double* __restrict__ dot_part, const int* skip) {
+  if (skip && *skip) return;
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double2 a = c[i], b = v[i];
+    acc += a.x * b.x + a.y * b.y;
+  }
+  const double sum = block_sum256(acc, sh);
+  if (threadIdx.x == 0) dot_part[blockIdx.x] = sum;
+}
+
 // ---- optional launch timing (used by the benchmark's roofline figure) ----
 // When enabled, every GEMM launch is bracketed by two HIP events on its own
 // stream and tagged with its template instantiation and algorithmic FLOP
@@ -418,6 +478,11 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
   const int64_t tiles = (int64_t)((g.M + kBM - 1) / kBM) * ((g.N + kBN - 1) / kBN);
   if (tiles > 0x7fffffff) {
     set_error("gemm: too many tiles");
+    return RENO_EINVAL;
+  }
+  if (g.dot_part && (g.batch != 1 || g.sCm != g.N || g.c_kind != ST_Z128 || !g.dot_v ||
+                     g.dot_np < 1)) {
+    set_error("gemm: fused dot needs one contiguous complex128 result");
     return RENO_EINVAL;
   }
   const bool ak = (g.sAk == 1);
@@ -479,7 +544,19 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
   // the timing bracket covers the matrix-core kernel only (the split-K
   // reduction is a separate, bandwidth-bound kernel)
   if (prof) (void)hipEventRecord(rec.b, stream);
-  if (nsplit > 1) {
+  const bool fuse_dot = g.dot_part != nullptr;
+  if (fuse_dot && nsplit > 1) {
+    const int64_t total = (int64_t)g.M * g.N;
+    if (g.prec == RENO_PREC_FP64)
+      hipLaunchKernelGGL(k_split_reduce_dot<double>, dim3(g.dot_np), dim3(256), 0, stream,
+                         (const double2*)g.part, nsplit, total, (double2*)g.C, g.beta, g.dot_v,
+                         g.dot_part, g.skip);
+    else
+      hipLaunchKernelGGL(k_split_reduce_dot<float>, dim3(g.dot_np), dim3(256), 0, stream,
+                         (const float2*)g.part, nsplit, total, (double2*)g.C, g.beta, g.dot_v,
+                         g.dot_part, g.skip);
+    RENO_CHECK_LAUNCH();
+  } else if (nsplit > 1) {
     const int64_t total = (int64_t)g.batch * g.M * g.N;
     const int blocks = (int)((total + 255) / 256 < 8192 ? (total + 255) / 256 : 8192);
     if (g.prec == RENO_PREC_FP64)
@@ -490,6 +567,11 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
       hipLaunchKernelGGL(k_split_reduce<float>, dim3(blocks), dim3(256), 0, stream,
                          (const float2*)g.part, nsplit, g.batch, g.M, g.N, g.C, g.c_kind, g.sCm,
                          g.sCb, g.beta, g.skip);
+    RENO_CHECK_LAUNCH();
+  }
+  if (fuse_dot && nsplit == 1) {
+    hipLaunchKernelGGL(k_dot_part, dim3(g.dot_np), dim3(256), 0, stream, (const double2*)g.C,
+                       g.dot_v, (int64_t)g.M * g.N, g.dot_part, g.skip);
     RENO_CHECK_LAUNCH();
   }
   if (prof) {

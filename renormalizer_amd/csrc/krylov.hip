@@ -58,23 +58,6 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return sh[0];
 }
 
-// partial[b] = sum over this block's grid-stride share of Re(conj(w) v)
-__global__ __launch_bounds__(kRedThreads) void k_dot_re(const double2* __restrict__ w,
-                                                        const double2* __restrict__ v,
-                                                        int64_t n, double* partial,
-                                                        const int* stop) {
-  if (*stop) return;
-  __shared__ double sh[kRedThreads];
-  double acc = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kRedThreads) {
-    const double2 a = w[i], b = v[i];
-    acc += a.x * b.x + a.y * b.y;
-  }
-  const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
 // partial[b] = sum |v|^2
 __global__ __launch_bounds__(kRedThreads) void k_norm2(const double2* __restrict__ v,
                                                        int64_t n, double* partial) {
@@ -89,19 +72,27 @@ __global__ __launch_bounds__(kRedThreads) void k_norm2(const double2* __restrict
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// w -= (alpha v_j + beta v_{j-1})  (krylov.py:70), partial[b] = sum |w|^2
-__global__ __launch_bounds__(kRedThreads) void k_recur_norm2(
+// alpha_j = Re<w, v_j> folded from the dot partials of the matvec (every
+// block folds them in the same order; block 0 stores alpha_j), then
+// w -= (alpha_j v_j + beta_{j-1} v_{j-1})  (krylov.py:57, 70) and
+// norm[b] = sum over the block's share of |w|^2
+__global__ __launch_bounds__(kRedThreads) void k_recur_alpha(
     double2* __restrict__ w, const double2* __restrict__ vj, const double2* __restrict__ vjm1,
-    const double* __restrict__ alpha_j, const double* __restrict__ beta_jm1, int64_t n,
-    double* partial, const int* stop) {
+    const double* __restrict__ dot_part, int np, double* __restrict__ alpha_j,
+    const double* __restrict__ beta_jm1, int64_t n, double* __restrict__ norm_part,
+    const int* stop) {
   if (*stop) return;
   __shared__ double sh[kRedThreads];
-  const double al = *alpha_j;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < np; i += kRedThreads) a += dot_part[i];
+  const double al = block_sum(a, sh);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *alpha_j = al;
+  __syncthreads();
   const double be = vjm1 ? *beta_jm1 : 0.0;
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kRedThreads) {
-    double2 a = w[i];
+    double2 x = w[i];
     const double2 b = vj[i];
     double tr = al * b.x, ti = al * b.y;
     if (vjm1) {
@@ -109,13 +100,13 @@ __global__ __launch_bounds__(kRedThreads) void k_recur_norm2(
       tr += be * c.x;
       ti += be * c.y;
     }
-    a.x -= tr;
-    a.y -= ti;
-    w[i] = a;
-    acc += a.x * a.x + a.y * a.y;
+    x.x -= tr;
+    x.y -= ti;
+    w[i] = x;
+    acc += x.x * x.x + x.y * x.y;
   }
-  const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  const double s2 = block_sum(acc, sh);
+  if (threadIdx.x == 0) norm_part[blockIdx.x] = s2;
 }
 
 // Start of a call: scal[0] = |v0|, state cleared; a zero v0 (the reference's
@@ -149,25 +140,34 @@ __global__ __launch_bounds__(kRedThreads) void k_fold_alpha(const double* __rest
   if (threadIdx.x == 0) out[0] = s;
 }
 
-// beta_j = sqrt(sum); breakdown (krylov.py:72-74) when beta_j < thr: the
-// result is the combination of the j + 1 vectors built so far
-__global__ __launch_bounds__(kRedThreads) void k_fold_beta(const double* __restrict__ partial,
-                                                           int np, double* out, double thr, int j,
-                                                           int* st) {
+// beta_j = sqrt(sum of the norm partials) (every block folds them in the
+// same order; block 0 stores beta_j and the state); breakdown (krylov.py:
+// 72-74) when beta_j < thr ends the iteration with the j + 1 vectors built so
+// far; otherwise V[j+1] = w / beta_j (vnext may be null: no slot left).
+__global__ __launch_bounds__(kRedThreads) void k_div_beta(const double2* __restrict__ w,
+                                                          double2* __restrict__ vnext,
+                                                          const double* __restrict__ norm_part,
+                                                          int np, double* __restrict__ beta_j,
+                                                          double thr, int j, int64_t n, int* st) {
   if (st[KS_STOP]) return;
   __shared__ double sh[kRedThreads];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += kRedThreads) acc += partial[i];
-  const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) {
-    const double v = sqrt(s);
-    out[0] = v;
-    if (v < thr) {
+  double a = 0.0;
+  for (int i = threadIdx.x; i < np; i += kRedThreads) a += norm_part[i];
+  const double be = sqrt(block_sum(a, sh));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *beta_j = be;
+    if (be < thr) {
       st[KS_STOP] = 1;
       st[KS_STATUS] = 2;
       st[KS_MFINAL] = j + 1;
       st[KS_NMATVEC] = j + 1;
     }
+  }
+  if (be < thr || !vnext) return;
+  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kRedThreads) {
+    const double2 x = w[i];
+    vnext[i] = make_double2(x.x / be, x.y / be);
   }
 }
 
@@ -383,8 +383,8 @@ __global__ void k_info(const int* st, int queued, int* info) {
 }
 
 struct KrylovLayout {
-  size_t off_wl, off_t1, off_t2, off_split, off_V, off_w, off_r0, off_r1, off_part, off_scal,
-      off_coef;
+  size_t off_wl, off_t1, off_t2, off_split, off_V, off_w, off_r0, off_r1, off_part, off_part2,
+      off_scal, off_coef;
   size_t split_bytes;
   size_t total;
 };
@@ -405,6 +405,7 @@ static KrylovLayout krylov_layout(const reno_hop_desc* op, int kcap) {
   k.off_r0 = o; o += vb;
   k.off_r1 = o; o += vb;
   k.off_part = o; o += align_up(kMaxPartials * sizeof(double));
+  k.off_part2 = o; o += align_up(kMaxPartials * sizeof(double));
   k.off_scal = o; o += align_up((size_t)(2 * kcap + 16) * sizeof(double));
   k.off_coef = o; o += align_up((size_t)kcap * 16);
   k.total = o;
@@ -436,6 +437,7 @@ struct KrylovRun {
   double2* w;
   double2* res[2];
   double* part;
+  double* part2;
   double* scal;
   int* st;
   double* d_alpha;
@@ -469,6 +471,7 @@ static void bind_run(KrylovRun& r, const reno_hop_desc* op, hipStream_t s, void*
   r.res[0] = (double2*)(base + r.L.off_r0);
   r.res[1] = (double2*)(base + r.L.off_r1);
   r.part = (double*)(base + r.L.off_part);
+  r.part2 = (double*)(base + r.L.off_part2);
   r.scal = (double*)(base + r.L.off_scal);  // [0] |v0|, [1] unused, [2..5] state ints
   r.st = (int*)(r.scal + 2);
   r.d_alpha = r.scal + 8;
@@ -515,23 +518,23 @@ static void launch_tri_coef(KrylovRun& r, int m, int final) {
 static int queue_iter(KrylovRun& r, int64_t j) {
   const int* stop = r.st + KS_STOP;
   double2* vj = r.V + (size_t)j * r.vstride;
-  RENO_TRY(hop_apply_core(r.op, r.wl, vj, r.w, r.t1, r.t2, r.s, stop, r.split, r.L.split_bytes));
-  hipLaunchKernelGGL(k_dot_re, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.w,
-                     (const double2*)vj, r.n, r.part, stop);
-  hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part,
-                     r.np, r.d_alpha + j, (const int*)r.st);
+  // matvec with the dot partials of Re<w, v_j> fused into its last stage
+  RENO_TRY(hop_apply_core(r.op, r.wl, vj, r.w, r.t1, r.t2, r.s, stop, r.split, r.L.split_bytes,
+                          vj, r.part, r.np));
   if (j == r.n - 1) {
+    hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part,
+                       r.np, r.d_alpha + j, (const int*)r.st);
     hipLaunchKernelGGL(k_mark_full, dim3(1), dim3(1), 0, r.s, (int)r.n, r.st);
     RENO_CHECK_LAUNCH();
     return RENO_OK;
   }
-  hipLaunchKernelGGL(k_recur_norm2, dim3(r.np), dim3(kRedThreads), 0, r.s, r.w,
-                     (const double2*)vj,
+  hipLaunchKernelGGL(k_recur_alpha, dim3(r.np), dim3(kRedThreads), 0, r.s, r.w, (const double2*)vj,
                      j > 0 ? (const double2*)(vj - r.vstride) : (const double2*)nullptr,
-                     (const double*)(r.d_alpha + j),
-                     (const double*)(j > 0 ? r.d_beta + j - 1 : r.d_beta), r.n, r.part, stop);
-  hipLaunchKernelGGL(k_fold_beta, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part,
-                     r.np, r.d_beta + j, r.thr, (int)j, r.st);
+                     (const double*)r.part, r.np, r.d_alpha + j,
+                     (const double*)(j > 0 ? r.d_beta + j - 1 : r.d_beta), r.n, r.part2, stop);
+  hipLaunchKernelGGL(k_div_beta, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.w,
+                     j + 1 < r.kcap ? r.V + (size_t)(j + 1) * r.vstride : (double2*)nullptr,
+                     (const double*)r.part2, r.np, r.d_beta + j, r.thr, (int)j, r.n, r.st);
   if (j > 3 && j % 2 == 0) {
     const int m = (int)j + 1;
     const int dst = (int)((j / 2) & 1);
@@ -545,10 +548,6 @@ static int queue_iter(KrylovRun& r, int64_t j) {
       hipLaunchKernelGGL(k_fold_check, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part,
                          r.np, (int)j, dst, r.st);
   }
-  if (j + 1 < r.kcap)
-    hipLaunchKernelGGL(k_div, dim3(r.ew_blocks), dim3(256), 0, r.s, (const double2*)r.w,
-                       r.V + (size_t)(j + 1) * r.vstride, (const double*)(r.d_beta + j), r.n,
-                       stop);
   RENO_CHECK_LAUNCH();
   return RENO_OK;
 }

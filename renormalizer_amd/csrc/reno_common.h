@@ -85,6 +85,12 @@ struct GemmCall {
   // by the timing hook (so launches cancelled through `skip` are not counted)
   int kchunk;
   int* exec;
+  // optional fused dot product with the result (complex128, batch 1,
+  // contiguous rows): dot_part[b] = partial sums of Re(sum conj(C) dot_v)
+  // over dot_np blocks (a fixed-order split; the caller folds them)
+  const double2* dot_v;
+  double* dot_part;
+  int dot_np;
 };
 
 int launch_gemm(const GemmCall& g, hipStream_t stream);

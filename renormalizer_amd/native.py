@@ -32,8 +32,8 @@ EXPORTS = (
     "reno_hop_workspace_bytes", "reno_krylov_workspace_bytes",
     "reno_env_workspace_bytes", "reno_qr_workspace_bytes",
     "reno_hop_apply", "reno_env_update", "reno_env_close", "reno_expm_krylov",
-    "reno_expm_krylov_async", "reno_block_qr", "reno_heff_diag", "reno_heff_dense", "reno_zgemm", "reno_probe_mfma16", "reno_prof_enable",
-    "reno_prof_collect",
+    "reno_expm_krylov_async", "reno_block_qr", "reno_heff_diag", "reno_heff_dense", "reno_zgemm",
+    "reno_probe_mfma16", "reno_prof_enable", "reno_prof_collect", "reno_prof_kind_info",
 )
 
 # demangled names of the GEMM kernel instantiations, indexed like
