@@ -64,19 +64,24 @@ def sigma_z(mps):
     return float((r[0, 0] - r[1, 1]).real)
 
 
+PROF_PERIOD = 16  # event-time a pseudo-random 1 in 16 of the GEMM launches
+
+
 def run_mode(mps0, mpo, precision, dt, warmup, steps, profile):
     import torch
-    from renormalizer_amd import EvolveConfig, EvolveMethod, native
+    from renormalizer_amd import EvolveConfig, EvolveMethod, Mps, native
     mps = mps0.copy()
     mps.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps, precision=precision)
     mps.to_device()
     history = [mps]
+    redo0 = Mps.krylov_redo_count
     for _ in range(warmup):
         mps = mps.evolve(mpo, dt)
         history.append(mps)
     torch.cuda.synchronize()
+    redo1 = Mps.krylov_redo_count
     if profile:
-        native.prof_enable(True)
+        native.prof_enable(PROF_PERIOD)
     t0 = time.perf_counter()
     for _ in range(steps):
         mps = mps.evolve(mpo, dt)
@@ -85,29 +90,55 @@ def run_mode(mps0, mpo, precision, dt, warmup, steps, profile):
     t1 = time.perf_counter()
     prof = None
     if profile:
-        native.prof_enable(False)
+        native.prof_enable(0)
         prof = native.prof_collect()
+        for name, p in sorted(prof.items(), key=lambda kv: -kv[1]["flop"]):
+            log(f"{precision}: gemm {name}: executed {p['executed']} queued {p['queued']} "
+                f"flop {p['flop']:.4g}; timed {p['timed']} in {p['timed_ms']:.1f} ms")
     matvecs = [sum(m.evolve_config.local_steps) for m in history[1 + warmup:]]
     sz = [sigma_z(m) for m in history]
     env = mps._env_cache[2] if mps._env_cache else None
-    from renormalizer_amd import Mps
-    log(f"{precision}: steps redone after an exhausted Krylov budget so far: {Mps.krylov_redo_count}")
+    redone = Mps.krylov_redo_count - redo1
+    log(f"{precision}: steps redone after an exhausted Krylov budget: warmup {redo1 - redo0}, "
+        f"timed {redone}")
     return {"seconds": t1 - t0, "steps_per_s": steps / (t1 - t0), "sz": sz, "prof": prof,
-            "matvecs_per_step": matvecs, "final": mps, "environ": env}
+            "matvecs_per_step": matvecs, "final": mps, "environ": env, "redone": redone}
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tdvp_c4_m256_fp64_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per executed launch of ``kernel`` from the committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (lines
+    "traffic_bytes_per_launch | <kernel> | <bytes>"), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = [p.strip() for p in line.strip().strip("|").split("|")]
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def roofline(prof, seconds, steps):
-    """Dominant GEMM instantiation: achieved TFLOP/s from summed algorithmic
-    FLOP over summed event-timed GPU time of its launches."""
-    name, (cnt, ms, fl) = max(prof.items(), key=lambda kv: kv[1][1])
+    """Dominant GEMM instantiation (most executed FLOP): achieved TFLOP/s =
+    algorithmic FLOP of its event-timed executed launches / their summed
+    event-timed GPU time."""
+    name, p = max(prof.items(), key=lambda kv: kv[1]["flop"])
     peak = PEAK_TFLOPS["double" if "<double" in name else "float"]
-    achieved = fl / (ms * 1e-3) / 1e12
-    total_flop = sum(v[2] for v in prof.values())
-    total_ms = sum(v[1] for v in prof.values())
+    achieved = p["timed_flop"] / (p["timed_ms"] * 1e-3) / 1e12
+    total_flop = sum(v["flop"] for v in prof.values())
+    # GEMM GPU time of all executed launches, scaled from each sample
+    total_ms = sum(v["timed_ms"] * v["executed"] / v["timed"] for v in prof.values() if v["timed"])
     return {
         "bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-        "frac": round(achieved / peak, 4), "traffic": None, "kernel": name,
-        "launches": cnt, "avg_launch_us": round(ms * 1e3 / cnt, 2),
+        "frac": round(achieved / peak, 4), "traffic": measured_traffic(name), "kernel": name,
+        "launches": p["executed"], "launches_queued": p["queued"], "launches_timed": p["timed"],
+        "avg_launch_us": round(p["timed_ms"] * 1e3 / p["timed"], 2),
+        "flop_per_launch": p["timed_flop"] / p["timed"],
         "gemm_flop_per_step": total_flop / steps,
         "gemm_time_fraction": round(total_ms * 1e-3 / seconds, 3),
         "step_tflops": round(total_flop / seconds / 1e12, 2),
@@ -158,6 +189,66 @@ def cpu_baseline(mps_dev, env, mpo, dt, sample_reps, gemm_flop_per_step):
     }
 
 
+def copy_calibration(mib=512, reps=3):
+    """A plain device-to-device copy of known size, for scaling the PMC
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    torch.cuda.set_device(0)
+    a = torch.empty(mib * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 8, "dispatches": reps}), flush=True)
+
+
+def _rocprof_csv(directory, suffix):
+    import csv
+    import glob
+    files = glob.glob(os.path.join(directory, "**", "*" + suffix), recursive=True)
+    if not files:
+        raise SystemExit(f"no *{suffix} under {directory}")
+    with open(files[0]) as f:
+        return list(csv.DictReader(f))
+
+
+def summarize_counters(directory, top=12):
+    """Per-kernel table of a rocprofv3 --pmc run: dispatches, dispatches
+    whose counter reads under 1 KiB (launches cancelled on the device by the
+    Krylov stop flag write nothing), counter sum and a log2(KiB) histogram."""
+    import collections
+    import math
+    agg = collections.defaultdict(list)
+    counter = None
+    for r in _rocprof_csv(directory, "counter_collection.csv"):
+        agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        counter = r.get("Counter_Name", counter)
+    print(f"counter {counter}")
+    print("| kernel | dispatches | under 1 KiB | sum KiB | log2(KiB) histogram |")
+    print("|---|---|---|---|---|")
+    for name, v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        hist = collections.Counter(int(math.log2(x)) if x >= 1 else -1 for x in v)
+        h = " ".join(f"{k}:{hist[k]}" for k in sorted(hist))
+        print(f"| {name} | {len(v)} | {sum(1 for x in v if x < 1)} | {sum(v):.1f} | {h} |")
+
+
+def summarize_trace(directory, top=12):
+    """Per-kernel table of a rocprofv3 --kernel-trace run: dispatches, total
+    and average duration, and the number of dispatches shorter than 2 us
+    (the launches cancelled on the device end within about a microsecond)."""
+    import collections
+    dur = collections.defaultdict(list)
+    for r in _rocprof_csv(directory, "kernel_trace.csv"):
+        dur[r["Kernel_Name"]].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    print("| kernel | dispatches | total ms | avg us | under 2 us |")
+    print("|---|---|---|---|---|")
+    for name, v in sorted(dur.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        print(f"| {name} | {len(v)} | {sum(v) / 1e6:.1f} | {sum(v) / len(v) / 1e3:.2f} | "
+              f"{sum(1 for x in v if x < 2000)} |")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,7 +261,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="run one precision mode only (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 512 MiB device copy (PMC counter calibration)")
+    ap.add_argument("--summarize-counters", metavar="DIR",
+                    help="print the per-kernel table of a rocprofv3 --pmc output directory")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="print the per-kernel table of a rocprofv3 --kernel-trace output directory")
     a = ap.parse_args()
+    if a.summarize_counters:
+        summarize_counters(a.summarize_counters)
+        return
+    if a.summarize_trace:
+        summarize_trace(a.summarize_trace)
+        return
+    if a.copy_calibration:
+        copy_calibration()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the TDVP-PS sweep of a single MPS "
@@ -223,6 +329,7 @@ def main():
                                f"omega_c=10 Wang1, M={a.bond_dim}, dt={a.dt}, TDVP-PS one-site",
                    "mode": main_mode, "sweep": "1 step = 2 half-sweeps"},
         "roofline": rl,
+        "krylov_steps_redone": r["redone"],
     }
     if err is not None:
         out["fp64_sweeps_per_s"] = round(res["fp64"]["steps_per_s"], 4)

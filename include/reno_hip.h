@@ -169,13 +169,24 @@ int reno_zgemm(int M, int N, int K, const void* A, const void* B, void* C,
                void* stream);
 
 /* ---- launch timing of the contraction kernel (no reference counterpart;
- * the benchmark's roofline figure).  reno_prof_enable(1) clears the record
- * and brackets every subsequent GEMM launch with HIP events on its stream;
- * reno_prof_enable(0) stops recording.  reno_prof_collect waits for the
- * events and fills, per kernel instantiation k (see reno_prof_kind_info),
- * the launch count, summed GPU milliseconds and summed algorithmic FLOP. */
-int reno_prof_enable(int on);
-int reno_prof_collect(int nkinds, double* ms, double* flop, int* count);
+ * the benchmark's roofline figure).  reno_prof_enable(period), period >= 1,
+ * clears the record and starts it: every subsequent GEMM launch is recorded
+ * (kernel instantiation, algorithmic FLOP, and a device-side marker that
+ * tells a launch that ran from one cancelled on the device), and a
+ * pseudo-random 1-in-period sample of them is bracketed by two HIP events
+ * on its stream (period 1: every launch; each event pair costs a few
+ * microseconds of stream time, so a sample keeps the measured run close to
+ * an unmeasured one).  reno_prof_enable(0) stops recording.
+ * reno_prof_collect waits for the events and fills, per kernel
+ * instantiation k (see reno_prof_kind_info): over the timed sample, the
+ * count of executed launches, their summed GPU milliseconds and summed
+ * algorithmic FLOP; over all recorded launches, the count and summed FLOP
+ * of the executed ones and the count of all, including those cancelled on
+ * the device (Lanczos iterations queued past convergence, which return at
+ * once).  Any output pointer may be NULL. */
+int reno_prof_enable(int period);
+int reno_prof_collect(int nkinds, double* ms, double* flop_timed, int* timed, double* flop,
+                      int* executed, int* queued);
 /* Number of GEMM kernel instantiations; for 0 <= kind < that number, fills
  * info[0..5] = precision, A k-contiguous, B k-contiguous, and the storage
  * kinds of A, B, C (0 complex128, 1 complex64, 2 float64, 3 float32). */

@@ -422,19 +422,23 @@ __global__ __launch_bounds__(256) void k_dot_part(const double2* __restrict__ c,
 }
 
 // ---- optional launch timing (used by the benchmark's roofline figure) ----
-// When enabled, every GEMM launch is bracketed by two HIP events on its own
-// stream and tagged with its template instantiation and algorithmic FLOP
-// count (8 per complex MAC, 4 per real x complex MAC); a per-launch marker
-// in device memory tells launches that ran from ones cancelled on the device.
+// When enabled, every GEMM launch is recorded with its template
+// instantiation and algorithmic FLOP count (8 per complex MAC, 4 per real x
+// complex MAC) and given a marker slot in device memory that the kernel sets
+// when it runs (launches cancelled on the device by the Krylov stop flag
+// leave it 0).  A pseudo-random 1-in-period sample of the launches is also
+// bracketed by two HIP events on its own stream.
 struct ProfRec {
-  hipEvent_t a, b;
+  hipEvent_t a, b;  // null unless the launch is in the timed sample
   double flop;
   int kind;
-  int slot;
+  int ran;  // marker value once harvested, -1 before
 };
 static std::mutex g_prof_mu;
-static bool g_prof_on = false;
+static int g_prof_period = 0;  // 0: off
+static uint64_t g_prof_seq = 0;
 static std::vector<ProfRec> g_prof;
+static size_t g_prof_harvested = 0;  // records [0, this) have `ran` set
 static std::vector<hipEvent_t> g_event_pool;
 static constexpr size_t kExecSlots = 1 << 20;
 static int* g_exec = nullptr;  // kExecSlots markers, allocated on first enable
@@ -448,6 +452,31 @@ static hipEvent_t pooled_event() {
   hipEvent_t e = nullptr;
   if (hipEventCreate(&e) != hipSuccess) return nullptr;
   return e;
+}
+
+// splitmix64 finaliser: the sample must not follow the periodic launch
+// pattern of the Lanczos iteration
+static uint64_t mix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+// Copy the markers of the records not yet harvested (at most kExecSlots of
+// them, in slot order) into their `ran` fields and clear the slots.  Waits
+// for the device.  Caller holds g_prof_mu.
+static int harvest_markers() {
+  const size_t n = g_prof.size() - g_prof_harvested;
+  if (n == 0 || !g_exec) return RENO_OK;
+  std::vector<int> ran(n);
+  RENO_HIP_CHECK(hipDeviceSynchronize());
+  RENO_HIP_CHECK(hipMemcpy(ran.data(), g_exec, n * sizeof(int), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) g_prof[g_prof_harvested + i].ran = ran[i];
+  g_prof_harvested += n;
+  RENO_HIP_CHECK(hipMemset(g_exec, 0, kExecSlots * sizeof(int)));
+  RENO_HIP_CHECK(hipDeviceSynchronize());
+  return RENO_OK;
 }
 
 int launch_gemm(const GemmCall& g, hipStream_t stream) {
@@ -521,20 +550,26 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
   }
   dim3 grid((unsigned)tiles, (unsigned)g.batch, (unsigned)nsplit);
   // K == 0 needs no special case: the kernel stores its zero accumulators.
-  ProfRec rec{nullptr, nullptr, 0.0, 0};
-  bool prof = false;
+  ProfRec rec{nullptr, nullptr, 0.0, 0, -1};
+  bool prof = false, timed = false;
   {
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    if (g_prof_on) {
-      rec.a = pooled_event();
-      rec.b = pooled_event();
-      prof = rec.a && rec.b;
-      if (prof && g_exec) {
-        k.exec = g_exec + (g_prof.size() % kExecSlots);
+    if (g_prof_period > 0 && g_exec) {
+      // every slot in use: collect the markers before reusing them
+      if (g_prof.size() - g_prof_harvested == kExecSlots) {
+        const int st = harvest_markers();
+        if (st != RENO_OK) return st;
+      }
+      prof = true;
+      k.exec = g_exec + (g_prof.size() - g_prof_harvested);
+      if (mix64(g_prof_seq++) % (uint64_t)g_prof_period == 0) {
+        rec.a = pooled_event();
+        rec.b = pooled_event();
+        timed = rec.a && rec.b;
       }
     }
   }
-  if (prof) (void)hipEventRecord(rec.a, stream);
+  if (timed) (void)hipEventRecord(rec.a, stream);
   {
     void* args[] = {(void*)&k};
     RENO_HIP_CHECK(hipLaunchKernel((const void*)kGemmKinds[kid].fn, grid, dim3(kThreads), args, 0,
@@ -543,7 +578,7 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
   RENO_CHECK_LAUNCH();
   // the timing bracket covers the matrix-core kernel only (the split-K
   // reduction is a separate, bandwidth-bound kernel)
-  if (prof) (void)hipEventRecord(rec.b, stream);
+  if (timed) (void)hipEventRecord(rec.b, stream);
   const bool fuse_dot = g.dot_part != nullptr;
   if (fuse_dot && nsplit > 1) {
     const int64_t total = (int64_t)g.M * g.N;
@@ -577,8 +612,8 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
   if (prof) {
     rec.flop = (areal ? 4.0 : 8.0) * (double)g.M * (double)g.N * (double)g.K * (double)g.batch;
     rec.kind = kid;
+    if (!timed) rec.a = rec.b = nullptr;
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    rec.slot = k.exec ? (int)(k.exec - g_exec) : -1;
     g_prof.push_back(rec);
   }
   return RENO_OK;
@@ -590,15 +625,17 @@ size_t splitk_bytes(int prec, int M, int N, int K, int batch) {
   return nsplit > 1 ? (size_t)nsplit * batch * M * N * elem_bytes(prec) : 0;
 }
 
-int prof_enable(int on) {
+int prof_enable(int period) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
-  g_prof_on = on != 0;
-  if (on) {
+  g_prof_period = period > 0 ? period : 0;
+  if (period > 0) {
     for (auto& r : g_prof) {
-      g_event_pool.push_back(r.a);
-      g_event_pool.push_back(r.b);
+      if (r.a) g_event_pool.push_back(r.a);
+      if (r.b) g_event_pool.push_back(r.b);
     }
     g_prof.clear();
+    g_prof_harvested = 0;
+    g_prof_seq = 0;
     if (!g_exec) RENO_HIP_CHECK(hipMalloc((void**)&g_exec, kExecSlots * sizeof(int)));
     RENO_HIP_CHECK(hipDeviceSynchronize());
     RENO_HIP_CHECK(hipMemset(g_exec, 0, kExecSlots * sizeof(int)));
@@ -607,32 +644,35 @@ int prof_enable(int on) {
   return RENO_OK;
 }
 
-// Sums, per instantiation, the launch count, GPU time (ms) and FLOP of the
-// recorded launches.  Waits for the recorded events.
-int prof_collect(int nkinds, double* ms, double* flop, int* count) {
+// Per instantiation: over the timed sample, executed launches, their GPU
+// milliseconds and FLOP; over all recorded launches, the executed count and
+// FLOP and the count of all.  Waits for the device.
+int prof_collect(int nkinds, double* ms, double* flop_timed, int* timed, double* flop,
+                 int* executed, int* queued) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
   for (int k = 0; k < nkinds; ++k) {
-    ms[k] = 0.0;
-    flop[k] = 0.0;
-    count[k] = 0;
+    if (ms) ms[k] = 0.0;
+    if (flop_timed) flop_timed[k] = 0.0;
+    if (timed) timed[k] = 0;
+    if (flop) flop[k] = 0.0;
+    if (executed) executed[k] = 0;
+    if (queued) queued[k] = 0;
   }
-  std::vector<int> executed;
-  if (g_exec && !g_prof.empty()) {
-    RENO_HIP_CHECK(hipDeviceSynchronize());
-    const size_t n = g_prof.size() < kExecSlots ? g_prof.size() : kExecSlots;
-    executed.resize(n);
-    RENO_HIP_CHECK(hipMemcpy(executed.data(), g_exec, n * sizeof(int), hipMemcpyDeviceToHost));
-  }
+  const int st = harvest_markers();
+  if (st != RENO_OK) return st;
   for (auto& r : g_prof) {
+    if (r.kind >= nkinds) continue;
+    if (queued) queued[r.kind] += 1;
+    if (!r.ran) continue;
+    if (executed) executed[r.kind] += 1;
+    if (flop) flop[r.kind] += r.flop;
+    if (!r.a) continue;
     RENO_HIP_CHECK(hipEventSynchronize(r.b));
-    if (r.slot >= 0 && (size_t)r.slot < executed.size() && !executed[r.slot]) continue;
     float t = 0.f;
     RENO_HIP_CHECK(hipEventElapsedTime(&t, r.a, r.b));
-    if (r.kind < nkinds) {
-      ms[r.kind] += t;
-      flop[r.kind] += r.flop;
-      count[r.kind] += 1;
-    }
+    if (ms) ms[r.kind] += t;
+    if (flop_timed) flop_timed[r.kind] += r.flop;
+    if (timed) timed[r.kind] += 1;
   }
   return RENO_OK;
 }
@@ -665,7 +705,7 @@ extern "C" int reno_probe_mfma16(int prec, const void* a, const void* b, void* c
   return RENO_OK;
 }
 
-extern "C" int reno_prof_enable(int on) { return reno::prof_enable(on); }
+extern "C" int reno_prof_enable(int period) { return reno::prof_enable(period); }
 
 extern "C" int reno_prof_kind_info(int kind, int* info) {
   if (kind >= 0 && kind < reno::kNumGemmKinds && info) {
@@ -680,8 +720,9 @@ extern "C" int reno_prof_kind_info(int kind, int* info) {
   return reno::kNumGemmKinds;
 }
 
-extern "C" int reno_prof_collect(int nkinds, double* ms, double* flop, int* count) {
-  return reno::prof_collect(nkinds, ms, flop, count);
+extern "C" int reno_prof_collect(int nkinds, double* ms, double* flop_timed, int* timed,
+                                 double* flop, int* executed, int* queued) {
+  return reno::prof_collect(nkinds, ms, flop_timed, timed, flop, executed, queued);
 }
 
 extern "C" int reno_zgemm(int M, int N, int K, const void* A, const void* B, void* C,

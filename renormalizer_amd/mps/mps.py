@@ -354,25 +354,31 @@ class Mps(MatrixProduct):
         # from self so that a second evolution of self rebuilds its own
         self._env_cache = None
         budgets = None
+        wide = True
         if reuse and self.evolve_config.krylov_async and cache[3] is not None \
                 and cache[3][0] == complex(evolve_dt):
             budgets = cache[3][1]
+            # counts of a step run with host waits (the first one, or a redo)
+            # move more than those of consecutive asynchronous steps
+            wide = cache[3][2]
         mps, environ, counts = self._tdvp_ps_sweep(mpo, evolve_dt, prec, cache[2] if reuse else None,
-                                                   budgets)
+                                                   budgets, wide)
+        synchronous = budgets is None
         if counts is None:
             # an iteration budget was too small: redo the step with host waits
             mps, environ, counts = self._tdvp_ps_sweep(mpo, evolve_dt, prec, None, None)
+            synchronous = True
         local_steps = [c for _, c in counts]
         mps.evolve_config.stat = stats.describe(local_steps)
         mps.evolve_config.local_steps = local_steps
-        mps._env_cache = (mpo, prec, environ, (complex(evolve_dt), dict(counts)))
+        mps._env_cache = (mpo, prec, environ, (complex(evolve_dt), dict(counts), synchronous))
         return mps
 
     # number of time steps redone with host waits because an asynchronous
     # Krylov budget proved too small (diagnostics)
     krylov_redo_count = 0
 
-    def _tdvp_ps_sweep(self, mpo, evolve_dt, prec, environ, budgets):
+    def _tdvp_ps_sweep(self, mpo, evolve_dt, prec, environ, budgets, wide=True):
         """The sweep itself.  With ``budgets`` (matvec counts of the previous
         step keyed by position) every Krylov call is queued asynchronously and
         the counts are read once at the end; returns counts None when a
@@ -397,10 +403,13 @@ class Mps(MatrixProduct):
                 out, j = native.expm_krylov(l_array, w, r_array, v, dt, prec)
                 counts.append((key, j))
                 return out
-            # the previous step's count at the same place, plus a quarter and
-            # four more convergence checks
+            # the previous step's count at the same place plus a margin: between
+            # consecutive asynchronous steps the counts move by 2 at the 99th
+            # percentile and by 6 at most (spin-boson, M = 256); after a step
+            # with host waits they move more
             prev = budgets.get(key, 0)
-            budget = max(prev + prev // 4 + 8, 12)
+            budget = prev + (prev // 4 + 8 if wide else prev // 16 + 6)
+            budget = max(budget, 12)
             out = native.expm_krylov_async(l_array, w, r_array, v, dt, prec, budget, info[len(keys)])
             keys.append(key)
             return out

@@ -126,7 +126,8 @@ def load_library(path: str = LIB_PATH):
         lib.reno_prof_kind_info.argtypes = [c_int, ctypes.POINTER(c_int)]
         lib.reno_prof_collect.restype = c_int
         lib.reno_prof_collect.argtypes = [c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_double),
-                                          ctypes.POINTER(c_int)]
+                                          ctypes.POINTER(c_int), ctypes.POINTER(c_double),
+                                          ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
         _lib = lib
         return lib
 
@@ -463,19 +464,29 @@ def zgemm(A, B):
     return C
 
 
-def prof_enable(on: bool):
-    check(lib().reno_prof_enable(int(bool(on))), "reno_prof_enable")
+def prof_enable(period: int):
+    """Start (period >= 1: time a pseudo-random 1-in-period sample of the
+    GEMM launches with HIP events) or stop (0) the GEMM launch record."""
+    check(lib().reno_prof_enable(int(period)), "reno_prof_enable")
 
 
 def prof_collect():
-    """{kernel name: (launches, total ms, total algorithmic FLOP)}."""
+    """{kernel name: {"timed", "timed_ms", "timed_flop": executed launches of
+    the event-timed sample, their GPU ms and algorithmic FLOP; "executed",
+    "flop": all executed launches and their FLOP; "queued": all launches,
+    including those cancelled on the device}}."""
     names = gemm_kind_names()
     n = len(names)
     ms = (ctypes.c_double * n)()
+    flt = (ctypes.c_double * n)()
+    tm = (ctypes.c_int * n)()
     fl = (ctypes.c_double * n)()
-    ct = (ctypes.c_int * n)()
-    check(lib().reno_prof_collect(n, ms, fl, ct), "reno_prof_collect")
-    return {names[k]: (ct[k], ms[k], fl[k]) for k in range(n) if ct[k]}
+    ex = (ctypes.c_int * n)()
+    qd = (ctypes.c_int * n)()
+    check(lib().reno_prof_collect(n, ms, flt, tm, fl, ex, qd), "reno_prof_collect")
+    return {names[k]: {"timed": tm[k], "timed_ms": ms[k], "timed_flop": flt[k], "executed": ex[k],
+                       "flop": fl[k], "queued": qd[k]}
+            for k in range(n) if qd[k]}
 
 
 def probe_mfma16(a, b, prec):
