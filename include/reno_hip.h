@@ -173,10 +173,10 @@ int reno_zgemm(int M, int N, int K, const void* A, const void* B, void* C,
  * clears the record and starts it: every subsequent GEMM launch is recorded
  * (kernel instantiation, algorithmic FLOP, and a device-side marker that
  * tells a launch that ran from one cancelled on the device), and a
- * pseudo-random 1-in-period sample of them is bracketed by two HIP events
- * on its stream (period 1: every launch; each event pair costs a few
- * microseconds of stream time, so a sample keeps the measured run close to
- * an unmeasured one).  reno_prof_enable(0) stops recording.
+ * pseudo-random 1-in-period sample of them is timed by two HIP events bound
+ * to the dispatch (hipExtLaunchKernel: kernel execution only, as a profiler
+ * reports it; period 1: every launch; timing every launch slows the stream,
+ * a sample keeps the measured run close to an unmeasured one).  reno_prof_enable(0) stops recording.
  * reno_prof_collect waits for the events and fills, per kernel
  * instantiation k (see reno_prof_kind_info): over the timed sample, the
  * count of executed launches, their summed GPU milliseconds and summed

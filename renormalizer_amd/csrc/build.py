@@ -21,10 +21,14 @@ BUILD_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # Packed FP32 instructions are disabled for all device code (they are not
-# needed: the contractions use MFMA) and the target is CDNA4 only.
+# needed: the contractions use MFMA) and the target is CDNA4 only.  Matrix-core
+# accumulators are kept in ordinary VGPRs: with the AGPR form the compiler
+# copied the GEMM accumulators between the two register files on every K step
+# and the doubled register footprint halved the waves per SIMD.
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wno-unused-result", "-munsafe-fp-atomics",
     "-I" + os.path.join(ROOT, "include"),
 ]

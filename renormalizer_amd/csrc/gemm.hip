@@ -17,6 +17,8 @@
 //   B operand: lane l holds B[l / 16][l % 16]
 //   C/D f32  : lane l, register i holds C[4 * (l / 16) + i][l % 16]
 //   C/D f64  : lane l, register i holds C[4 * i + l / 16][l % 16]
+#include <hip/hip_ext.h>
+
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -187,7 +189,7 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
       if (gm < g.M && gk < ke)
         load_k<R, AKIND>(g.A, aoff + (int64_t)gm * g.sAm + (int64_t)gk * g.sAk, re, im);
       pa_re[i] = re;
-      pa_im[i] = g.conjA ? -im : im;
+      pa_im[i] = im;
       int nn;
       b_coord(e, kk, nn);
       const int gn = n0 + nn;
@@ -197,10 +199,15 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
       if (gn < g.N && gk2 < ke)
         load_k<R, BKIND>(g.B, boff + (int64_t)gk2 * g.sBk + (int64_t)gn * g.sBn, re, im);
       pb_re[i] = re;
-      pb_im[i] = g.conjB ? -im : im;
+      pb_im[i] = im;
     }
   };
 
+  // conjugation is applied here, after the multiply of the current tile:
+  // touching the prefetched registers any earlier would make the wave wait
+  // for the global loads before its matrix-core work
+  const R sgnA = g.conjA ? (R)-1 : (R)1;
+  const R sgnB = g.conjB ? (R)-1 : (R)1;
   auto sstore = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i) {
@@ -208,11 +215,11 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
       int mm, kk;
       a_coord(e, mm, kk);
       sA[buf][0][kk][mm] = pa_re[i];
-      if (!AREAL) sA[buf][1][kk][mm] = pa_im[i];
+      if (!AREAL) sA[buf][1][kk][mm] = sgnA * pa_im[i];
       int nn;
       b_coord(e, kk, nn);
       sB[buf][0][kk][nn] = pb_re[i];
-      sB[buf][1][kk][nn] = pb_im[i];
+      sB[buf][1][kk][nn] = sgnB * pb_im[i];
     }
   };
 
@@ -427,7 +434,7 @@ __global__ __launch_bounds__(256) void k_dot_part(const double2* __restrict__ c,
 // complex MAC) and given a marker slot in device memory that the kernel sets
 // when it runs (launches cancelled on the device by the Krylov stop flag
 // leave it 0).  A pseudo-random 1-in-period sample of the launches is also
-// bracketed by two HIP events on its own stream.
+// timed by two HIP events bound to the dispatch (hipExtLaunchKernel).
 struct ProfRec {
   hipEvent_t a, b;  // null unless the launch is in the timed sample
   double flop;
@@ -569,16 +576,20 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
       }
     }
   }
-  if (timed) (void)hipEventRecord(rec.a, stream);
   {
     void* args[] = {(void*)&k};
-    RENO_HIP_CHECK(hipLaunchKernel((const void*)kGemmKinds[kid].fn, grid, dim3(kThreads), args, 0,
-                                   stream));
+    if (timed) {
+      // the events are bound to this dispatch: they time the matrix-core
+      // kernel alone, without the queue latency in front of it (the split-K
+      // reduction is a separate, bandwidth-bound kernel)
+      RENO_HIP_CHECK(hipExtLaunchKernel((const void*)kGemmKinds[kid].fn, grid, dim3(kThreads), args,
+                                        0, stream, rec.a, rec.b, 0));
+    } else {
+      RENO_HIP_CHECK(hipLaunchKernel((const void*)kGemmKinds[kid].fn, grid, dim3(kThreads), args, 0,
+                                     stream));
+    }
   }
   RENO_CHECK_LAUNCH();
-  // the timing bracket covers the matrix-core kernel only (the split-K
-  // reduction is a separate, bandwidth-bound kernel)
-  if (timed) (void)hipEventRecord(rec.b, stream);
   const bool fuse_dot = g.dot_part != nullptr;
   if (fuse_dot && nsplit > 1) {
     const int64_t total = (int64_t)g.M * g.N;

@@ -142,8 +142,12 @@ class MatrixProduct:
     def _get_sigmaqn(self, idx):
         raise NotImplementedError
 
-    def _get_big_qn(self, cidx: List[int]):
-        """Super-block quantum numbers of the active site(s)."""
+    def _get_big_qn(self, cidx: List[int], with_qnmat: bool = True):
+        """Super-block quantum numbers of the active site(s) (reference
+        mp.py:308-352).  The third value, the quantum numbers of every
+        coefficient (rows x cols), is None unless ``with_qnmat``: at M = 256
+        it is a 655 360-entry array, built per site update for nothing on the
+        time-evolution path."""
         if len(cidx) == 2:
             cidx = sorted(cidx)
             assert cidx[0] + 1 == cidx[1]
@@ -161,7 +165,7 @@ class MatrixProduct:
         else:
             qnbigl = add_outer(qnl, sigmaqn[0])
             qnbigr = add_outer(sigmaqn[1], qnr)
-        return qnbigl, qnbigr, add_outer(qnbigl, qnbigr)
+        return qnbigl, qnbigr, add_outer(qnbigl, qnbigr) if with_qnmat else None
 
     # ---- copies ----------------------------------------------------------
     def metacopy(self):
@@ -316,7 +320,7 @@ class MatrixProduct:
     def _push_cano(self, idx):
         mt = self[idx]
         assert np.any(mt)
-        qnbigl, qnbigr, _ = self._get_big_qn([idx])
+        qnbigl, qnbigr, _ = self._get_big_qn([idx], with_qnmat=False)
         system = "L" if self.to_right else "R"
         u, qnlset, v, qnrset = svd_qn_host(mt, qnbigl, qnbigr, self.qntot, QR=True, system=system,
                                            full_matrices=False)
@@ -387,7 +391,7 @@ class MatrixProduct:
         system = "L" if self.to_right else "R"
         s_list = []
         for idx in self.iter_idx_list(full=False):
-            qnbigl, qnbigr, _ = self._get_big_qn([idx])
+            qnbigl, qnbigr, _ = self._get_big_qn([idx], with_qnmat=False)
             u, sigma, qnlset, v, sigma, qnrset = svd_qn_host(self[idx], qnbigl, qnbigr, self.qntot,
                                                              system=system, full_matrices=False)
             s_list.append(sigma)

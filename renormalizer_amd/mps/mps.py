@@ -422,7 +422,7 @@ class Mps(MatrixProduct):
                 site = mps._mp[imps]
                 shape = list(site.shape)
                 mps_t = krylov((half, imps, 1), l_array, wt[imps], r_array, site, -1j * evolve_dt / 2)
-                qnbigl, qnbigr, _ = mps._get_big_qn([imps])
+                qnbigl, qnbigr, _ = mps._get_big_qn([imps], with_qnmat=False)
                 u, qnlset, vt, qnrset = svd_qn_device(mps_t, qnbigl, qnbigr, mps.qntot, system)
                 if not mps.to_right and imps != 0:
                     kt = vt.shape[0]
