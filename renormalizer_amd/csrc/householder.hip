@@ -148,14 +148,20 @@ __device__ void apply_prev_column(double2* __restrict__ X, int m, int ld, int c,
 // kept small: a fully unrolled panel overflows the instruction cache).  The
 // registers hold the not-yet-factored columns rotated so that column jj is
 // always slot 0 (slot c = column jj + c); after each step the slots shift
-// left by one, so every register index is a compile-time constant.  One
-// reduction per column: |x|^2 and g_c = sum_{r>j} conj(a_rj) a_rc; the pivot
-// row (alpha, a_jc) goes through LDS (double-buffered by column parity).
-// Each column is written to X / Vall as soon as it is final.
+// left by one and the new reflector v_jj enters slot 7, so every register
+// index is a compile-time constant and, at step jj, v_q (q < jj) sits in
+// slot 8 - jj + q.  One reduction per column gives |x|^2, the dot products
+// g_c = sum_{r>j} conj(a_rj) a_rc with the later columns, and the same sums
+// with the earlier reflectors, from which column jj of V^H V and hence
+// column jj of the compact-WY factor T (zlarft, forward columnwise) follow
+// without another pass over V.  The pivot row (alpha, a_jc and the v_q(j))
+// goes through LDS (double-buffered by column parity).  Each column is
+// written to X / Vall as soon as it is final.
 template <int RPT>
 __device__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, int m, int ld, int j0, int jb,
                              double2* __restrict__ X, double2* __restrict__ Vall, double* sh,
-                             double* red, double2 (*piv)[kQrNB], double2* sTau) {
+                             double* red, double2 (*piv)[kQrNB], double2* sTau,
+                             double2 (*sT)[kQrNB]) {
   const int tid = threadIdx.x;
 #pragma unroll 1
   for (int jj = 0; jj < jb; ++jj) {
@@ -191,13 +197,32 @@ __device__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, int m, int ld, in
     if (!(xnorm2 == 0.0 && alpha.y == 0.0)) {
       const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
       beta = alpha.x >= 0.0 ? -nrm : nrm;
-      tj = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+      const double ib = 1.0 / beta;
+      tj = make_double2((beta - alpha.x) * ib, -alpha.y * ib);
       // scale = 1 / (alpha - beta)
       const double dr = alpha.x - beta, di = alpha.y;
-      const double den = dr * dr + di * di;
-      scale = make_double2(dr / den, -di / den);
+      const double iden = 1.0 / (dr * dr + di * di);
+      scale = make_double2(dr * iden, -di * iden);
     }
     if (tid == 0) sTau[jj] = tj;
+    // T[q][jj] = -tau_jj sum_{p=q}^{jj-1} T[q][p] S[p][jj] for q < jj, with
+    // S[p][jj] = v_p^H v_jj = conj(v_p(j)) + scale conj(G_p), G_p the sum of
+    // conj(x) v_p over the rows below j (slot 8 - jj + p of the reduction)
+    if (tid < jj) {
+      const int q = tid;
+      double2 acc = make_double2(0.0, 0.0);
+      for (int p = q; p < jj; ++p) {
+        const int sl = kQrNB - jj + p;
+        const double2 gp = make_double2(red[1 + 2 * sl], -red[2 + 2 * sl]);  // conj(G_p)
+        const double2 vpj = pv[sl];
+        const double2 spj = cadd(make_double2(vpj.x, -vpj.y), cmul(scale, gp));
+        acc = cadd(acc, cmul(sT[q][p], spj));
+      }
+      const double2 v = cmul(tj, acc);
+      sT[q][jj] = make_double2(-v.x, -v.y);
+    } else if (tid == jj) {
+      sT[jj][jj] = tj;
+    }
     const bool refl = !(tj.x == 0.0 && tj.y == 0.0);
     // f_c = conj(tau) (a_jc + conj(scale) g_c): coefficient of v in H^H a_c
     double2 f[kQrNB];
@@ -223,14 +248,15 @@ __device__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, int m, int ld, in
         X[r + (int64_t)j * ld] = out;
         Vall[r + (int64_t)j * m] = r < j ? make_double2(0.0, 0.0) : vr;
       }
-      // later columns: a_c -= v f_c on rows >= j, then shift left
+      // later columns: a_c -= v f_c on rows >= j, then shift left (f_c is
+      // zero for the reflector slots); v_jj enters the last slot
 #pragma unroll
       for (int c = 1; c < kQrNB; ++c) {
         double2 y = a[i][c];
         if (r >= j) y = csub(y, cmul(vr, f[c]));
         a[i][c - 1] = y;
       }
-      a[i][kQrNB - 1] = make_double2(0.0, 0.0);
+      a[i][kQrNB - 1] = (r < j || r >= m) ? make_double2(0.0, 0.0) : vr;
     }
   }
 }
@@ -246,41 +272,6 @@ __device__ __forceinline__ void load_panel(double2 (&a)[RPT][kQrNB], const doubl
 #pragma unroll
     for (int c = 0; c < kQrNB; ++c)
       a[i][c] = (r < m && c < jb) ? X[r + (int64_t)(j0 + c) * ld] : make_double2(0.0, 0.0);
-  }
-}
-
-// sS[q][c] = v_q^H v_c for q < c, c in [C0, C1), v from the explicit V just
-// written (rows j0.. of columns j0..j0+jb-1; zero above the diagonal).
-template <int C0, int C1>
-__device__ __forceinline__ void panel_gram(const double2* __restrict__ Vall, int m, int j0, int jb,
-                                           double* sh, double* red, double2 (*sS)[kQrNB]) {
-  constexpr int NP = (C1 * (C1 - 1) - C0 * (C0 - 1)) / 2;  // pairs q < c, C0 <= c < C1
-  const int tid = threadIdx.x;
-  double s[2 * NP];
-#pragma unroll
-  for (int t = 0; t < 2 * NP; ++t) s[t] = 0.0;
-#pragma unroll 1
-  for (int r = j0 + tid; r < m; r += kPanelThreads) {
-    double2 v[C1];
-#pragma unroll
-    for (int c = 0; c < C1; ++c)
-      v[c] = c < jb ? Vall[r + (int64_t)(j0 + c) * m] : make_double2(0.0, 0.0);
-    int t = 0;
-#pragma unroll
-    for (int c = C0; c < C1; ++c) {
-#pragma unroll
-      for (int q = 0; q < c; ++q, ++t) {
-        const double2 p = cmulc(v[q], v[c]);
-        s[2 * t] += p.x;
-        s[2 * t + 1] += p.y;
-      }
-    }
-  }
-  block_reduce<2 * NP>(s, sh, red);
-  if (tid == 0) {
-    int t = 0;
-    for (int c = C0; c < C1; ++c)
-      for (int q = 0; q < c; ++q, ++t) sS[q][c] = make_double2(red[2 * t], red[2 * t + 1]);
   }
 }
 
@@ -315,7 +306,6 @@ __global__ __launch_bounds__(kPanelThreads) void k_qr_step(double2* __restrict__
   __shared__ double sh[(kPanelThreads / 64) * 64];
   __shared__ double red[64];
   __shared__ double2 sT[kQrNB][kQrNB];
-  __shared__ double2 sS[kQrNB][kQrNB];
   __shared__ double2 sTau[kQrNB];
   __shared__ double2 sPiv[2][kQrNB];
   __shared__ double2 sF[kQrNB * kQrNB];
@@ -330,32 +320,12 @@ __global__ __launch_bounds__(kPanelThreads) void k_qr_step(double2* __restrict__
     return;
   }
   if (jb == 0) return;
+  for (int t = tid; t < kQrNB * kQrNB; t += kPanelThreads) sT[t / kQrNB][t % kQrNB] = make_double2(0.0, 0.0);
   double2 a[RPT][kQrNB];
   load_panel<RPT>(a, X, m, ld, j0, jb);
-  factor_panel<RPT>(a, j0, m, ld, j0, jb, X, Vall, sh, red, sPiv, sTau);
+  factor_panel<RPT>(a, j0, m, ld, j0, jb, X, Vall, sh, red, sPiv, sTau, sT);
   for (int r = tid; r < j0; r += kPanelThreads)
     for (int c = 0; c < jb; ++c) Vall[r + (int64_t)(j0 + c) * m] = make_double2(0.0, 0.0);
-  // ---- S[q][c] = v_q^H v_c (q < c) from the explicit V just written (each
-  // thread reads back its own rows), in two reductions; then T (zlarft,
-  // forward, columnwise) by thread 0
-  panel_gram<1, 5>(Vall, m, j0, jb, sh, red, sS);
-  panel_gram<5, kQrNB>(Vall, m, j0, jb, sh, red, sS);
-  lds_barrier();
-  if (tid == 0) {
-    for (int q = 0; q < kQrNB; ++q)
-      for (int c = 0; c < kQrNB; ++c) sT[q][c] = make_double2(0.0, 0.0);
-    for (int c = 0; c < jb; ++c) {
-      const double2 tc = sTau[c];
-      sT[c][c] = tc;
-      // T[0:c, c] = -tau_c T[0:c, 0:c] S[0:c, c]
-      for (int q = 0; q < c; ++q) {
-        double2 acc = make_double2(0.0, 0.0);
-        for (int p = q; p < c; ++p) acc = cadd(acc, cmul(sT[q][p], sS[p][c]));
-        const double2 v = cmul(tc, acc);
-        sT[q][c] = make_double2(-v.x, -v.y);
-      }
-    }
-  }
   lds_barrier();
   double2* Tb = Tall + (size_t)(j0 / kQrNB) * 2 * kQrNB * kQrNB;
   if (tid < kQrNB * kQrNB) {
