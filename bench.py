@@ -68,40 +68,49 @@ PROF_PERIOD = 16  # event-time a pseudo-random 1 in 16 of the GEMM launches
 
 
 def run_mode(mps0, mpo, precision, dt, warmup, steps, profile):
+    """Evolve a copy of mps0 by warmup + steps time steps.  Each timed step is
+    bracketed by device synchronisations and only the evolution itself is
+    timed; <sigma_z> is taken after each step outside the timed region, and
+    only the current state is kept (earlier states are released, so the
+    allocator recycles their memory as a long run would)."""
     import torch
     from renormalizer_amd import EvolveConfig, EvolveMethod, Mps, native
     mps = mps0.copy()
     mps.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps, precision=precision)
     mps.to_device()
-    history = [mps]
+    sz = [sigma_z(mps)]
     redo0 = Mps.krylov_redo_count
     for _ in range(warmup):
         mps = mps.evolve(mpo, dt)
-        history.append(mps)
+        sz.append(sigma_z(mps))
     torch.cuda.synchronize()
     redo1 = Mps.krylov_redo_count
-    if profile:
-        native.prof_enable(PROF_PERIOD)
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    seconds = 0.0
+    matvecs = []
+    for k in range(steps):
+        if profile:
+            # a new record for the first step, resumed for the later ones
+            native.prof_enable(PROF_PERIOD if k == 0 else -PROF_PERIOD)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         mps = mps.evolve(mpo, dt)
-        history.append(mps)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+        torch.cuda.synchronize()
+        seconds += time.perf_counter() - t0
+        if profile:
+            native.prof_enable(0)
+        matvecs.append(sum(mps.evolve_config.local_steps))
+        sz.append(sigma_z(mps))
     prof = None
     if profile:
-        native.prof_enable(0)
         prof = native.prof_collect()
         for name, p in sorted(prof.items(), key=lambda kv: -kv[1]["flop"]):
             log(f"{precision}: gemm {name}: executed {p['executed']} queued {p['queued']} "
                 f"flop {p['flop']:.4g}; timed {p['timed']} in {p['timed_ms']:.1f} ms")
-    matvecs = [sum(m.evolve_config.local_steps) for m in history[1 + warmup:]]
-    sz = [sigma_z(m) for m in history]
     env = mps._env_cache[2] if mps._env_cache else None
     redone = Mps.krylov_redo_count - redo1
     log(f"{precision}: steps redone after an exhausted Krylov budget: warmup {redo1 - redo0}, "
         f"timed {redone}")
-    return {"seconds": t1 - t0, "steps_per_s": steps / (t1 - t0), "sz": sz, "prof": prof,
+    return {"seconds": seconds, "steps_per_s": steps / seconds, "sz": sz, "prof": prof,
             "matvecs_per_step": matvecs, "final": mps, "environ": env, "redone": redone}
 
 

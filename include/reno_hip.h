@@ -176,7 +176,9 @@ int reno_zgemm(int M, int N, int K, const void* A, const void* B, void* C,
  * pseudo-random 1-in-period sample of them is timed by two HIP events bound
  * to the dispatch (hipExtLaunchKernel: kernel execution only, as a profiler
  * reports it; period 1: every launch; timing every launch slows the stream,
- * a sample keeps the measured run close to an unmeasured one).  reno_prof_enable(0) stops recording.
+ * a sample keeps the measured run close to an unmeasured one).  reno_prof_enable(0) stops recording;
+ * reno_prof_enable(-period) resumes it without clearing (work between a stop
+ * and a resume is not recorded).
  * reno_prof_collect waits for the events and fills, per kernel
  * instantiation k (see reno_prof_kind_info): over the timed sample, the
  * count of executed launches, their summed GPU milliseconds and summed

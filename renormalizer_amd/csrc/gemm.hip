@@ -328,22 +328,33 @@ static int find_kind(const GemmCall& g, bool ak, bool bk) {
 
 // ---- split-K -------------------------------------------------------------
 // Split K into chunks (multiples of the K tile) when the output tile grid
-// would leave most of the 256 CUs idle; each chunk writes its own partial and
+// would leave the chip idle; each chunk writes its own partial and
 // k_split_reduce sums them in a fixed order (deterministic, no atomics).
+// The split minimises an estimate: rounds of resident workgroups (two per
+// CU: 512 on the 256 CUs) times the K extent of one workgroup plus a fixed
+// per-workgroup cost (prologue, tile store) of about 48 K steps; chunks stay
+// at least 64 deep.
 static void split_plan(int M, int N, int K, int batch, int* nsplit, int* kchunk) {
   const int64_t tiles = (int64_t)((M + kBM - 1) / kBM) * ((N + kBN - 1) / kBN) * batch;
   *nsplit = 1;
   *kchunk = K > 0 ? K : 1;
-  const int64_t target = 512;  // about two workgroups per CU
-  if (tiles >= target / 2 || K < 256) return;
-  int s = (int)((target + tiles - 1) / tiles);
-  const int max_s = K / 128;  // keep >= 128 of K per chunk
-  if (s > max_s) s = max_s;
-  if (s < 2) return;
-  int chunk = (K + s - 1) / s;
-  chunk = (chunk + kBK - 1) / kBK * kBK;
-  *kchunk = chunk;
-  *nsplit = (K + chunk - 1) / chunk;
+  constexpr int64_t kSlots = 512;
+  constexpr int kMinChunk = 64;
+  constexpr int64_t kFixed = 48;
+  if (tiles >= kSlots * 7 / 8 || K < 2 * kMinChunk) return;
+  auto est = [&](int64_t ns, int64_t chunk) { return ((tiles * ns + kSlots - 1) / kSlots) * (chunk + kFixed); };
+  int64_t best = est(1, K);
+  for (int s = 2; s <= K / kMinChunk; ++s) {
+    int chunk = (K + s - 1) / s;
+    chunk = (chunk + kBK - 1) / kBK * kBK;
+    const int ns = (K + chunk - 1) / chunk;
+    const int64_t e = est(ns, chunk);
+    if (e < best) {
+      best = e;
+      *nsplit = ns;
+      *kchunk = chunk;
+    }
+  }
 }
 
 // C[b](m,n) (+)= sum_s P[s][b][m][n]
@@ -638,7 +649,15 @@ size_t splitk_bytes(int prec, int M, int N, int K, int batch) {
 
 int prof_enable(int period) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
-  g_prof_period = period > 0 ? period : 0;
+  if (period < 0) {  // resume the current record
+    if (!g_exec) {
+      set_error("prof_enable: nothing to resume");
+      return RENO_EINVAL;
+    }
+    g_prof_period = -period;
+    return RENO_OK;
+  }
+  g_prof_period = period;
   if (period > 0) {
     for (auto& r : g_prof) {
       if (r.a) g_event_pool.push_back(r.a);

@@ -1,8 +1,13 @@
 // Lanczos approximation of expm(dt*H) v (reference lib/krylov/krylov.py:15-84).
 //
 // The three-term recurrence runs on the device in FP64 with deterministic
-// two-pass reductions (per-block partial sums, then one block folds them in a
-// fixed order), so repeated calls give bit-identical results.
+// two-pass reductions (per-block partial sums folded in a fixed order, by the
+// last block of the same launch or by the next kernel), so repeated calls
+// give bit-identical results.  Krylov vectors are stored unnormalised: slot
+// j holds V[j] = v_j / sig_j with sig_j = 1 / beta_{j-1} kept beside alpha
+// and beta, and every consumer applies the factor (the matvec is linear, so
+// w = H V[j] is scaled afterwards).  The recurrence thereby writes the next
+// vector in the same pass that computes its norm.
 //
 // The reference's control flow runs on the device too: the breakdown test
 // (beta_j < 100 n eps), the full-space exit, and at every even j > 3 the
@@ -37,6 +42,7 @@ enum {
   KS_MFINAL = 2,   // Krylov vectors of the final combination (status 2, 3)
   KS_RESIDX = 3,   // result buffer holding the converged vector (status 1)
   KS_NMATVEC = 4,  // matvecs used (reference j + 1)
+  KS_COUNT = 5,    // blocks finished in the current launch (last-block folds)
   KS_INTS = 8
 };
 
@@ -72,47 +78,75 @@ __global__ __launch_bounds__(kRedThreads) void k_norm2(const double2* __restrict
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// alpha_j = Re<w, v_j> folded from the dot partials of the matvec (every
-// block folds them in the same order; block 0 stores alpha_j), then
-// w -= (alpha_j v_j + beta_{j-1} v_{j-1})  (krylov.py:57, 70) and
-// norm[b] = sum over the block's share of |w|^2
-__global__ __launch_bounds__(kRedThreads) void k_recur_alpha(
-    double2* __restrict__ w, const double2* __restrict__ vj, const double2* __restrict__ vjm1,
-    const double* __restrict__ dot_part, int np, double* __restrict__ alpha_j,
-    const double* __restrict__ beta_jm1, int64_t n, double* __restrict__ norm_part,
-    const int* stop) {
-  if (*stop) return;
+// Iteration j of the recurrence (krylov.py:57, 70-74), with w = H V[j]:
+// alpha_j = Re<H v_j, v_j> = sig_j^2 Re<w, V[j]> folded from the dot partials
+// of the matvec (every block folds them in the same order; block 0 stores
+// alpha_j), then
+//   V[j+1] = sig_j w - alpha_j sig_j V[j] - beta_{j-1} sig_{j-1} V[j-1]
+// (= beta_j v_{j+1}) into vnext, with per-block partial sums of |V[j+1]|^2.
+// The last block to finish folds them: beta_j, sig_{j+1} = 1 / beta_j, and
+// the breakdown test (beta_j < thr) that ends the iteration with the j + 1
+// vectors built so far.
+__global__ __launch_bounds__(kRedThreads) void k_recur(
+    const double2* __restrict__ w, const double2* __restrict__ vj, const double2* __restrict__ vjm1,
+    double2* vnext, const double* __restrict__ dot_part, int np_dot, double* __restrict__ alpha,
+    double* __restrict__ beta, double* __restrict__ sig, int j, int64_t n, double thr,
+    double* norm_part, int* st) {
+  if (st[KS_STOP]) return;
   __shared__ double sh[kRedThreads];
+  __shared__ int last;
   double a = 0.0;
-  for (int i = threadIdx.x; i < np; i += kRedThreads) a += dot_part[i];
-  const double al = block_sum(a, sh);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *alpha_j = al;
+  for (int i = threadIdx.x; i < np_dot; i += kRedThreads) a += dot_part[i];
+  const double sj = sig[j];
+  const double al = block_sum(a, sh) * sj * sj;
+  if (blockIdx.x == 0 && threadIdx.x == 0) alpha[j] = al;
   __syncthreads();
-  const double be = vjm1 ? *beta_jm1 : 0.0;
+  const double ca = al * sj;
+  const double cb = vjm1 ? beta[j - 1] * sig[j - 1] : 0.0;
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kRedThreads) {
-    double2 x = w[i];
+    const double2 x = w[i];
     const double2 b = vj[i];
-    double tr = al * b.x, ti = al * b.y;
+    double xr = sj * x.x - ca * b.x, xi = sj * x.y - ca * b.y;
     if (vjm1) {
       const double2 c = vjm1[i];
-      tr += be * c.x;
-      ti += be * c.y;
+      xr -= cb * c.x;
+      xi -= cb * c.y;
     }
-    x.x -= tr;
-    x.y -= ti;
-    w[i] = x;
-    acc += x.x * x.x + x.y * x.y;
+    vnext[i] = make_double2(xr, xi);
+    acc += xr * xr + xi * xi;
   }
   const double s2 = block_sum(acc, sh);
-  if (threadIdx.x == 0) norm_part[blockIdx.x] = s2;
+  if (threadIdx.x == 0) {
+    norm_part[blockIdx.x] = s2;
+    __threadfence();
+    last = atomicAdd(st + KS_COUNT, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double b2 = 0.0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += kRedThreads) b2 += norm_part[i];
+  const double be = sqrt(block_sum(b2, sh));
+  if (threadIdx.x == 0) {
+    st[KS_COUNT] = 0;
+    beta[j] = be;
+    sig[j + 1] = 1.0 / be;
+    if (be < thr) {
+      st[KS_STOP] = 1;
+      st[KS_STATUS] = 2;
+      st[KS_MFINAL] = j + 1;
+      st[KS_NMATVEC] = j + 1;
+    }
+  }
 }
 
 // Start of a call: scal[0] = |v0|, state cleared; a zero v0 (the reference's
 // assert, krylov.py:44) ends the call with status -1.
 __global__ __launch_bounds__(kRedThreads) void k_kry_init(const double* __restrict__ partial,
-                                                          int np, double* scal, int* st) {
+                                                          int np, double* scal, double* sig,
+                                                          int* st) {
   __shared__ double sh[kRedThreads];
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += kRedThreads) acc += partial[i];
@@ -122,6 +156,7 @@ __global__ __launch_bounds__(kRedThreads) void k_kry_init(const double* __restri
   if (threadIdx.x == 0) {
     const double nrm = sqrt(s);
     scal[0] = nrm;
+    sig[0] = 1.0;
     if (!(nrm > 0.0)) {
       st[KS_STOP] = 1;
       st[KS_STATUS] = -1;
@@ -129,46 +164,16 @@ __global__ __launch_bounds__(kRedThreads) void k_kry_init(const double* __restri
   }
 }
 
-// alpha_j = sum of partials
+// alpha_j = sig_j^2 (sum of the dot partials) (last possible iteration)
 __global__ __launch_bounds__(kRedThreads) void k_fold_alpha(const double* __restrict__ partial,
-                                                            int np, double* out, const int* st) {
+                                                            int np, double* out,
+                                                            const double* sig_j, const int* st) {
   if (st[KS_STOP]) return;
   __shared__ double sh[kRedThreads];
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += kRedThreads) acc += partial[i];
   const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) out[0] = s;
-}
-
-// beta_j = sqrt(sum of the norm partials) (every block folds them in the
-// same order; block 0 stores beta_j and the state); breakdown (krylov.py:
-// 72-74) when beta_j < thr ends the iteration with the j + 1 vectors built so
-// far; otherwise V[j+1] = w / beta_j (vnext may be null: no slot left).
-__global__ __launch_bounds__(kRedThreads) void k_div_beta(const double2* __restrict__ w,
-                                                          double2* __restrict__ vnext,
-                                                          const double* __restrict__ norm_part,
-                                                          int np, double* __restrict__ beta_j,
-                                                          double thr, int j, int64_t n, int* st) {
-  if (st[KS_STOP]) return;
-  __shared__ double sh[kRedThreads];
-  double a = 0.0;
-  for (int i = threadIdx.x; i < np; i += kRedThreads) a += norm_part[i];
-  const double be = sqrt(block_sum(a, sh));
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *beta_j = be;
-    if (be < thr) {
-      st[KS_STOP] = 1;
-      st[KS_STATUS] = 2;
-      st[KS_MFINAL] = j + 1;
-      st[KS_NMATVEC] = j + 1;
-    }
-  }
-  if (be < thr || !vnext) return;
-  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kRedThreads) {
-    const double2 x = w[i];
-    vnext[i] = make_double2(x.x / be, x.y / be);
-  }
+  if (threadIdx.x == 0) out[0] = s * sig_j[0] * sig_j[0];
 }
 
 // Krylov space equals the full space after iteration n-1 (krylov.py:59-61)
@@ -178,22 +183,6 @@ __global__ void k_mark_full(int n, int* st) {
   st[KS_STATUS] = 3;
   st[KS_MFINAL] = n;
   st[KS_NMATVEC] = n;
-}
-
-// allclose check at iteration j: no violations -> converged, result in res[residx]
-__global__ __launch_bounds__(kRedThreads) void k_fold_check(const double* __restrict__ partial,
-                                                            int np, int j, int residx, int* st) {
-  if (st[KS_STOP]) return;
-  __shared__ double sh[kRedThreads];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += kRedThreads) acc += partial[i];
-  const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0 && s == 0.0) {
-    st[KS_STOP] = 1;
-    st[KS_STATUS] = 1;
-    st[KS_RESIDX] = residx;
-    st[KS_NMATVEC] = j + 1;
-  }
 }
 
 // dst = src / (*den)
@@ -303,27 +292,42 @@ __global__ __launch_bounds__(64) void k_tri_coef(const double* __restrict__ alph
   }
 }
 
-// res_new[i] = sum_j coef[j] V[j][i] and the count of elements violating
-// |res_old - res_new| <= atol + rtol |res_new| (numpy.allclose(res_old,
-// res_new), krylov.py:76-81); res_old may be null (first check: no count).
+// res_new[i] = sum_j coef[j] sig_j V[j][i] and the count of elements
+// violating |res_old - res_new| <= atol + rtol |res_new| (numpy.allclose(
+// res_old, res_new), krylov.py:76-81); res_old may be null (first check: no
+// test).  The last block to finish folds the per-block counts: none means
+// converged at iteration j with the result in res_new (buffer residx).
 __global__ __launch_bounds__(kRedThreads) void k_combine(const double2* __restrict__ V,
                                                          int64_t n, int64_t vstride, int nv,
                                                          const double2* __restrict__ coef,
+                                                         const double* __restrict__ sig,
                                                          const double2* __restrict__ res_old,
                                                          double2* __restrict__ res_new,
-                                                         double* partial, const int* st) {
+                                                         double* partial, int j, int residx,
+                                                         int* st) {
   if (st[KS_STOP]) return;
   __shared__ double sh[kRedThreads];
   __shared__ double2 sc[64];
-  for (int j = threadIdx.x; j < nv && j < 64; j += kRedThreads) sc[j] = coef[j];
+  __shared__ int last;
+  for (int k = threadIdx.x; k < nv && k < 64; k += kRedThreads) {
+    const double2 c = coef[k];
+    sc[k] = make_double2(c.x * sig[k], c.y * sig[k]);
+  }
   __syncthreads();
   double viol = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kRedThreads) {
     double xr = 0.0, xi = 0.0;
-    for (int j = 0; j < nv; ++j) {
-      const double2 c = j < 64 ? sc[j] : coef[j];
-      const double2 v = V[(int64_t)j * vstride + i];
+    for (int k = 0; k < nv; ++k) {
+      double2 c;
+      if (k < 64) {
+        c = sc[k];
+      } else {
+        c = coef[k];
+        c.x *= sig[k];
+        c.y *= sig[k];
+      }
+      const double2 v = V[(int64_t)k * vstride + i];
       xr += v.x * c.x - v.y * c.y;
       xi += v.x * c.y + v.y * c.x;
     }
@@ -336,9 +340,27 @@ __global__ __launch_bounds__(kRedThreads) void k_combine(const double2* __restri
       if (!(lhs <= rhs)) viol += 1.0;
     }
   }
-  if (res_old) {
-    const double s = block_sum(viol, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (!res_old) return;
+  const double sv = block_sum(viol, sh);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = sv;
+    __threadfence();
+    last = atomicAdd(st + KS_COUNT, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += kRedThreads) acc += partial[i];
+  const double tot = block_sum(acc, sh);
+  if (threadIdx.x == 0) {
+    st[KS_COUNT] = 0;
+    if (tot == 0.0) {
+      st[KS_STOP] = 1;
+      st[KS_STATUS] = 1;
+      st[KS_RESIDX] = residx;
+      st[KS_NMATVEC] = j + 1;
+    }
   }
 }
 
@@ -348,6 +370,7 @@ __global__ __launch_bounds__(kRedThreads) void k_combine(const double2* __restri
 __global__ __launch_bounds__(kRedThreads) void k_final_out(const double2* __restrict__ V,
                                                            int64_t n, int64_t vstride,
                                                            const double2* __restrict__ coef,
+                                                           const double* __restrict__ sig,
                                                            const double2* __restrict__ res0,
                                                            const double2* __restrict__ res1,
                                                            double2* __restrict__ out,
@@ -364,7 +387,7 @@ __global__ __launch_bounds__(kRedThreads) void k_final_out(const double2* __rest
          i += (int64_t)gridDim.x * kRedThreads) {
       double xr = 0.0, xi = 0.0;
       for (int j = 0; j < nv; ++j) {
-        const double2 c = coef[j];
+        const double2 c = make_double2(coef[j].x * sig[j], coef[j].y * sig[j]);
         const double2 v = V[(int64_t)j * vstride + i];
         xr += v.x * c.x - v.y * c.y;
         xi += v.x * c.y + v.y * c.x;
@@ -406,7 +429,7 @@ static KrylovLayout krylov_layout(const reno_hop_desc* op, int kcap) {
   k.off_r1 = o; o += vb;
   k.off_part = o; o += align_up(kMaxPartials * sizeof(double));
   k.off_part2 = o; o += align_up(kMaxPartials * sizeof(double));
-  k.off_scal = o; o += align_up((size_t)(2 * kcap + 16) * sizeof(double));
+  k.off_scal = o; o += align_up((size_t)(3 * kcap + 17) * sizeof(double));
   k.off_coef = o; o += align_up((size_t)kcap * 16);
   k.total = o;
   return k;
@@ -442,6 +465,7 @@ struct KrylovRun {
   int* st;
   double* d_alpha;
   double* d_beta;
+  double* d_sig;  // kcap + 1 entries
   double2* d_coef;
   int np, ew_blocks;
   double thr, dt_re, dt_im;
@@ -476,6 +500,7 @@ static void bind_run(KrylovRun& r, const reno_hop_desc* op, hipStream_t s, void*
   r.st = (int*)(r.scal + 2);
   r.d_alpha = r.scal + 8;
   r.d_beta = r.scal + 8 + kcap;
+  r.d_sig = r.scal + 8 + 2 * kcap;
   r.d_coef = (double2*)(base + r.L.off_coef);
   r.np = n_partials(r.n);
   r.ew_blocks = (int)((r.n + 255) / 256 < 2048 ? (r.n + 255) / 256 : 2048);
@@ -491,7 +516,7 @@ static int queue_start(KrylovRun& r, const void* v0) {
   hipLaunchKernelGGL(k_norm2, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)v0, r.n,
                      r.part);
   hipLaunchKernelGGL(k_kry_init, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part, r.np,
-                     r.scal, r.st);
+                     r.scal, r.d_sig, r.st);
   hipLaunchKernelGGL(k_div, dim3(r.ew_blocks), dim3(256), 0, r.s, (const double2*)v0, r.V,
                      (const double*)r.scal, r.n, (const int*)r.st);
   RENO_CHECK_LAUNCH();
@@ -518,35 +543,32 @@ static void launch_tri_coef(KrylovRun& r, int m, int final) {
 static int queue_iter(KrylovRun& r, int64_t j) {
   const int* stop = r.st + KS_STOP;
   double2* vj = r.V + (size_t)j * r.vstride;
-  // matvec with the dot partials of Re<w, v_j> fused into its last stage
+  // matvec with the dot partials of Re<w, V[j]> fused into its last stage
   RENO_TRY(hop_apply_core(r.op, r.wl, vj, r.w, r.t1, r.t2, r.s, stop, r.split, r.L.split_bytes,
                           vj, r.part, r.np));
   if (j == r.n - 1) {
     hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part,
-                       r.np, r.d_alpha + j, (const int*)r.st);
+                       r.np, r.d_alpha + j, (const double*)(r.d_sig + j), (const int*)r.st);
     hipLaunchKernelGGL(k_mark_full, dim3(1), dim3(1), 0, r.s, (int)r.n, r.st);
     RENO_CHECK_LAUNCH();
     return RENO_OK;
   }
-  hipLaunchKernelGGL(k_recur_alpha, dim3(r.np), dim3(kRedThreads), 0, r.s, r.w, (const double2*)vj,
-                     j > 0 ? (const double2*)(vj - r.vstride) : (const double2*)nullptr,
-                     (const double*)r.part, r.np, r.d_alpha + j,
-                     (const double*)(j > 0 ? r.d_beta + j - 1 : r.d_beta), r.n, r.part2, stop);
-  hipLaunchKernelGGL(k_div_beta, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.w,
-                     j + 1 < r.kcap ? r.V + (size_t)(j + 1) * r.vstride : (double2*)nullptr,
-                     (const double*)r.part2, r.np, r.d_beta + j, r.thr, (int)j, r.n, r.st);
+  // no slot for V[j+1] (the last queued iteration of a full workspace): the
+  // recurrence still runs for beta_j and the breakdown test, into w
+  double2* vnext = j + 1 < r.kcap ? r.V + (size_t)(j + 1) * r.vstride : r.w;
+  hipLaunchKernelGGL(k_recur, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.w,
+                     (const double2*)vj, j > 0 ? (const double2*)(vj - r.vstride) : (const double2*)nullptr,
+                     vnext, (const double*)r.part, r.np, r.d_alpha, r.d_beta, r.d_sig, (int)j, r.n,
+                     r.thr, r.part2, r.st);
   if (j > 3 && j % 2 == 0) {
     const int m = (int)j + 1;
     const int dst = (int)((j / 2) & 1);
     const bool compare = j > 4;
     launch_tri_coef(r, m, 0);
     hipLaunchKernelGGL(k_combine, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.V, r.n,
-                       (int64_t)r.vstride, m, (const double2*)r.d_coef,
+                       (int64_t)r.vstride, m, (const double2*)r.d_coef, (const double*)r.d_sig,
                        compare ? (const double2*)r.res[dst ^ 1] : (const double2*)nullptr,
-                       r.res[dst], r.part, (const int*)r.st);
-    if (compare)
-      hipLaunchKernelGGL(k_fold_check, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part,
-                         r.np, (int)j, dst, r.st);
+                       r.res[dst], r.part, (int)j, dst, r.st);
   }
   RENO_CHECK_LAUNCH();
   return RENO_OK;
@@ -556,7 +578,7 @@ static int queue_iter(KrylovRun& r, int64_t j) {
 static int queue_final(KrylovRun& r, void* out, int m_max) {
   launch_tri_coef(r, m_max, 1);
   hipLaunchKernelGGL(k_final_out, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.V,
-                     r.n, (int64_t)r.vstride, (const double2*)r.d_coef,
+                     r.n, (int64_t)r.vstride, (const double2*)r.d_coef, (const double*)r.d_sig,
                      (const double2*)r.res[0], (const double2*)r.res[1], (double2*)out,
                      (const int*)r.st);
   RENO_CHECK_LAUNCH();

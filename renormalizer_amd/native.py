@@ -466,7 +466,8 @@ def zgemm(A, B):
 
 def prof_enable(period: int):
     """Start (period >= 1: time a pseudo-random 1-in-period sample of the
-    GEMM launches with HIP events) or stop (0) the GEMM launch record."""
+    GEMM launches with HIP events), stop (0) or resume without clearing
+    (-period) the GEMM launch record."""
     check(lib().reno_prof_enable(int(period)), "reno_prof_enable")
 
 
