@@ -78,31 +78,60 @@ __global__ __launch_bounds__(kRedThreads) void k_norm2(const double2* __restrict
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// beta_{j-1} = |V[j]| folded from the norm partials the previous recurrence
+// left (every block folds them in the same order), sig_j = 1 / beta_{j-1};
+// block 0 stores both.  Breakdown (beta_{j-1} < thr, krylov.py:72-74) ends
+// the iteration with the j vectors built before it (the matvec of V[j]
+// queued meanwhile is discarded).  Returns false on breakdown.
+__device__ __forceinline__ bool fold_beta(const double* __restrict__ norm_part, int np, int j,
+                                          double thr, double* __restrict__ beta,
+                                          double* __restrict__ sig, int* st, double* sh,
+                                          double* be_out) {
+  double b2 = 0.0;
+  for (int i = threadIdx.x; i < np; i += kRedThreads) b2 += norm_part[i];
+  const double be = sqrt(block_sum(b2, sh));
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    beta[j - 1] = be;
+    sig[j] = 1.0 / be;
+    if (be < thr) {
+      st[KS_STOP] = 1;
+      st[KS_STATUS] = 2;
+      st[KS_MFINAL] = j;
+      st[KS_NMATVEC] = j;
+    }
+  }
+  *be_out = be;
+  return !(be < thr);
+}
+
 // Iteration j of the recurrence (krylov.py:57, 70-74), with w = H V[j]:
+// sig_j from the previous iteration's norm partials (fold_beta; sig_0 = 1),
 // alpha_j = Re<H v_j, v_j> = sig_j^2 Re<w, V[j]> folded from the dot partials
 // of the matvec (every block folds them in the same order; block 0 stores
 // alpha_j), then
 //   V[j+1] = sig_j w - alpha_j sig_j V[j] - beta_{j-1} sig_{j-1} V[j-1]
-// (= beta_j v_{j+1}) into vnext, with per-block partial sums of |V[j+1]|^2.
-// The last block to finish folds them: beta_j, sig_{j+1} = 1 / beta_j, and
-// the breakdown test (beta_j < thr) that ends the iteration with the j + 1
-// vectors built so far.
+// (= beta_j v_{j+1}) into vnext, and per-block partial sums of |V[j+1]|^2
+// for the next iteration.
 __global__ __launch_bounds__(kRedThreads) void k_recur(
     const double2* __restrict__ w, const double2* __restrict__ vj, const double2* __restrict__ vjm1,
-    double2* vnext, const double* __restrict__ dot_part, int np_dot, double* __restrict__ alpha,
-    double* __restrict__ beta, double* __restrict__ sig, int j, int64_t n, double thr,
-    double* norm_part, int* st) {
+    double2* vnext, const double* __restrict__ dot_part, const double* __restrict__ norm_prev,
+    int np, double* __restrict__ alpha, double* __restrict__ beta, double* __restrict__ sig,
+    int j, int64_t n, double thr, double* __restrict__ norm_part, int* st) {
   if (st[KS_STOP]) return;
   __shared__ double sh[kRedThreads];
-  __shared__ int last;
+  double bprev = 1.0;  // beta_{j-1}
+  if (j > 0 && !fold_beta(norm_prev, np, j, thr, beta, sig, st, sh, &bprev)) return;
+  const double sj = j > 0 ? 1.0 / bprev : 1.0;
   double a = 0.0;
-  for (int i = threadIdx.x; i < np_dot; i += kRedThreads) a += dot_part[i];
-  const double sj = sig[j];
+  for (int i = threadIdx.x; i < np; i += kRedThreads) a += dot_part[i];
   const double al = block_sum(a, sh) * sj * sj;
   if (blockIdx.x == 0 && threadIdx.x == 0) alpha[j] = al;
   __syncthreads();
   const double ca = al * sj;
-  const double cb = vjm1 ? beta[j - 1] * sig[j - 1] : 0.0;
+  // beta_{j-1} was folded above (block 0 stores it concurrently, so it is
+  // not read back); sig_{j-1} was stored by an earlier launch
+  const double cb = vjm1 ? bprev * sig[j - 1] : 0.0;
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kRedThreads) {
@@ -118,28 +147,7 @@ __global__ __launch_bounds__(kRedThreads) void k_recur(
     acc += xr * xr + xi * xi;
   }
   const double s2 = block_sum(acc, sh);
-  if (threadIdx.x == 0) {
-    norm_part[blockIdx.x] = s2;
-    __threadfence();
-    last = atomicAdd(st + KS_COUNT, 1) == (int)gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double b2 = 0.0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += kRedThreads) b2 += norm_part[i];
-  const double be = sqrt(block_sum(b2, sh));
-  if (threadIdx.x == 0) {
-    st[KS_COUNT] = 0;
-    beta[j] = be;
-    sig[j + 1] = 1.0 / be;
-    if (be < thr) {
-      st[KS_STOP] = 1;
-      st[KS_STATUS] = 2;
-      st[KS_MFINAL] = j + 1;
-      st[KS_NMATVEC] = j + 1;
-    }
-  }
+  if (threadIdx.x == 0) norm_part[blockIdx.x] = s2;
 }
 
 // Start of a call: scal[0] = |v0|, state cleared; a zero v0 (the reference's
@@ -164,16 +172,24 @@ __global__ __launch_bounds__(kRedThreads) void k_kry_init(const double* __restri
   }
 }
 
-// alpha_j = sig_j^2 (sum of the dot partials) (last possible iteration)
+// Last possible iteration j = n - 1 (one block): beta_{j-1} / sig_j with the
+// breakdown test as in k_recur, then alpha_j = sig_j^2 (sum of the dot
+// partials).
 __global__ __launch_bounds__(kRedThreads) void k_fold_alpha(const double* __restrict__ partial,
-                                                            int np, double* out,
-                                                            const double* sig_j, const int* st) {
+                                                            const double* __restrict__ norm_prev,
+                                                            int np, double* __restrict__ alpha,
+                                                            double* __restrict__ beta,
+                                                            double* __restrict__ sig, int j,
+                                                            double thr, int* st) {
   if (st[KS_STOP]) return;
   __shared__ double sh[kRedThreads];
+  double bprev = 1.0;
+  if (j > 0 && !fold_beta(norm_prev, np, j, thr, beta, sig, st, sh, &bprev)) return;
+  const double sj = j > 0 ? 1.0 / bprev : 1.0;
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += kRedThreads) acc += partial[i];
   const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) out[0] = s * sig_j[0] * sig_j[0];
+  if (threadIdx.x == 0) alpha[j] = s * sj * sj;
 }
 
 // Krylov space equals the full space after iteration n-1 (krylov.py:59-61)
@@ -428,7 +444,7 @@ static KrylovLayout krylov_layout(const reno_hop_desc* op, int kcap) {
   k.off_r0 = o; o += vb;
   k.off_r1 = o; o += vb;
   k.off_part = o; o += align_up(kMaxPartials * sizeof(double));
-  k.off_part2 = o; o += align_up(kMaxPartials * sizeof(double));
+  k.off_part2 = o; o += align_up(2 * kMaxPartials * sizeof(double));
   k.off_scal = o; o += align_up((size_t)(3 * kcap + 17) * sizeof(double));
   k.off_coef = o; o += align_up((size_t)kcap * 16);
   k.total = o;
@@ -548,7 +564,8 @@ static int queue_iter(KrylovRun& r, int64_t j) {
                           vj, r.part, r.np));
   if (j == r.n - 1) {
     hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part,
-                       r.np, r.d_alpha + j, (const double*)(r.d_sig + j), (const int*)r.st);
+                       (const double*)(r.part2 + (size_t)(j & 1) * kMaxPartials), r.np, r.d_alpha,
+                       r.d_beta, r.d_sig, (int)j, r.thr, r.st);
     hipLaunchKernelGGL(k_mark_full, dim3(1), dim3(1), 0, r.s, (int)r.n, r.st);
     RENO_CHECK_LAUNCH();
     return RENO_OK;
@@ -556,10 +573,14 @@ static int queue_iter(KrylovRun& r, int64_t j) {
   // no slot for V[j+1] (the last queued iteration of a full workspace): the
   // recurrence still runs for beta_j and the breakdown test, into w
   double2* vnext = j + 1 < r.kcap ? r.V + (size_t)(j + 1) * r.vstride : r.w;
+  // the norm partials of V[j+1] go to the other half of part2 (the current
+  // half holds those of V[j], still being folded by every block)
+  double* nprev = r.part2 + (size_t)(j & 1) * kMaxPartials;
+  double* nnext = r.part2 + (size_t)((j + 1) & 1) * kMaxPartials;
   hipLaunchKernelGGL(k_recur, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.w,
                      (const double2*)vj, j > 0 ? (const double2*)(vj - r.vstride) : (const double2*)nullptr,
-                     vnext, (const double*)r.part, r.np, r.d_alpha, r.d_beta, r.d_sig, (int)j, r.n,
-                     r.thr, r.part2, r.st);
+                     vnext, (const double*)r.part, (const double*)nprev, r.np, r.d_alpha, r.d_beta,
+                     r.d_sig, (int)j, r.n, r.thr, nnext, r.st);
   if (j > 3 && j % 2 == 0) {
     const int m = (int)j + 1;
     const int dst = (int)((j / 2) & 1);
