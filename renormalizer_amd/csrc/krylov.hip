@@ -311,20 +311,19 @@ __global__ __launch_bounds__(64) void k_tri_coef(const double* __restrict__ alph
 // res_new[i] = sum_j coef[j] sig_j V[j][i] and the count of elements
 // violating |res_old - res_new| <= atol + rtol |res_new| (numpy.allclose(
 // res_old, res_new), krylov.py:76-81); res_old may be null (first check: no
-// test).  The last block to finish folds the per-block counts: none means
-// converged at iteration j with the result in res_new (buffer residx).
+// test).  The last block to finish sees whether any block counted a
+// violation: none means converged at iteration j with the result in res_new
+// (buffer residx).
 __global__ __launch_bounds__(kRedThreads) void k_combine(const double2* __restrict__ V,
                                                          int64_t n, int64_t vstride, int nv,
                                                          const double2* __restrict__ coef,
                                                          const double* __restrict__ sig,
                                                          const double2* __restrict__ res_old,
-                                                         double2* __restrict__ res_new,
-                                                         double* partial, int j, int residx,
-                                                         int* st) {
+                                                         double2* __restrict__ res_new, int j,
+                                                         int residx, int* st) {
   if (st[KS_STOP]) return;
   __shared__ double sh[kRedThreads];
   __shared__ double2 sc[64];
-  __shared__ int last;
   for (int k = threadIdx.x; k < nv && k < 64; k += kRedThreads) {
     const double2 c = coef[k];
     sc[k] = make_double2(c.x * sig[k], c.y * sig[k]);
@@ -357,25 +356,21 @@ __global__ __launch_bounds__(kRedThreads) void k_combine(const double2* __restri
     }
   }
   if (!res_old) return;
+  // the verdict travels in the same atomic as the completion count (low 12
+  // bits: blocks done, the rest: blocks with a violation), so no partial sums
+  // and no device-wide fence are needed
   const double sv = block_sum(viol, sh);
   if (threadIdx.x == 0) {
-    partial[blockIdx.x] = sv;
-    __threadfence();
-    last = atomicAdd(st + KS_COUNT, 1) == (int)gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += kRedThreads) acc += partial[i];
-  const double tot = block_sum(acc, sh);
-  if (threadIdx.x == 0) {
-    st[KS_COUNT] = 0;
-    if (tot == 0.0) {
-      st[KS_STOP] = 1;
-      st[KS_STATUS] = 1;
-      st[KS_RESIDX] = residx;
-      st[KS_NMATVEC] = j + 1;
+    const int inc = 1 + (sv > 0.0 ? (1 << 12) : 0);
+    const int tot = atomicAdd(st + KS_COUNT, inc) + inc;
+    if ((tot & ((1 << 12) - 1)) == (int)gridDim.x) {
+      st[KS_COUNT] = 0;
+      if ((tot >> 12) == 0) {
+        st[KS_STOP] = 1;
+        st[KS_STATUS] = 1;
+        st[KS_RESIDX] = residx;
+        st[KS_NMATVEC] = j + 1;
+      }
     }
   }
 }
@@ -589,7 +584,7 @@ static int queue_iter(KrylovRun& r, int64_t j) {
     hipLaunchKernelGGL(k_combine, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.V, r.n,
                        (int64_t)r.vstride, m, (const double2*)r.d_coef, (const double*)r.d_sig,
                        compare ? (const double2*)r.res[dst ^ 1] : (const double2*)nullptr,
-                       r.res[dst], r.part, (int)j, dst, r.st);
+                       r.res[dst], (int)j, dst, r.st);
   }
   RENO_CHECK_LAUNCH();
   return RENO_OK;
