@@ -46,8 +46,10 @@ enum {
   KS_INTS = 8
 };
 
+// one block per 256 elements up to kMaxPartials blocks: the bond-matrix
+// vectors (65 536 elements at M = 256) still spread over 256 CUs
 static int n_partials(int64_t n) {
-  int64_t b = (n + kRedThreads * 4 - 1) / (kRedThreads * 4);
+  int64_t b = (n + kRedThreads - 1) / kRedThreads;
   if (b < 1) b = 1;
   if (b > kMaxPartials) b = kMaxPartials;
   return (int)b;
