@@ -197,3 +197,53 @@ def test_async_budget_redo_matches_sync():
     ref = sync.evolve(mpo, 0.5)
     assert nxt.evolve_config.local_steps == ref.evolve_config.local_steps
     np.testing.assert_allclose(_sz_dev(nxt), _sz_dev(ref), atol=1e-10)
+
+
+def test_holstein_c2_shape_parity():
+    """The C2 workload shape: Holstein chain of 16 molecules in scheme 2
+    (32 sites, electron d = 2, phonon d = 10, omega = 1, g = 1, J = -1),
+    Franck-Condon electron on molecule 8, M = 64, dt = 0.02, one-electron
+    quantum numbers.  First 3 steps, FP64 device vs oracle: populations to
+    1e-8, matvec counts, quantum numbers bit-identical."""
+    print("[tdvp C2 holstein 32 sites M=64, 3 steps]", flush=True)
+    model = helpers.holstein_model(16, 10, j=-1.0, g=1.0)
+    mps, mpo = helpers.holstein_initial(model, 64)
+    assert max(mps.bond_dims) == 64
+    obs_d = lambda m: m.e_occupations
+    obs_o = lambda s: helpers.e_occupations_oracle(s, model)
+    od, oo, sd, so, dmps, st = _run_pair(mps, mpo, 0.02, 3, obs_d, obs_o)
+    np.testing.assert_allclose(od, oo, atol=1e-8, rtol=0)
+    np.testing.assert_allclose(od.sum(axis=1), 1.0, atol=1e-10)
+    _assert_steps_match(sd, so)
+    assert [np.array(q).tolist() for q in dmps.qn] == [np.array(q).tolist() for q in st.qn]
+
+
+def test_spin_boson_c5_shape_parity():
+    """The C5 workload shape: spin + 32 modes x 8 levels, alpha = 0.4,
+    omega_c = 5, M = 64, dt = 0.05.  First 2 steps: FP64 device vs oracle to
+    1e-8 with the matvec counts; the mixed mode (complex64 environments, FP32
+    matrix cores) within 1e-4 of the FP64 oracle on <sigma_z>."""
+    print("[tdvp C5 sbm 33 sites M=64, 2 steps]", flush=True)
+    model = helpers.sbm_model(32, 8, alpha=0.4, omega_c=5.0)
+    mps, mpo = helpers.sbm_initial(model, 64)
+    st = helpers.to_state(mps)
+    w = helpers.mpo_arrays(mpo)
+    runs = {}
+    for precision in ("fp64", "mixed"):
+        m = mps.copy()
+        m.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps, precision=precision)
+        m.to_device()
+        sz, steps = [_sz_dev(m)], []
+        for _ in range(2):
+            m = m.evolve(mpo, 0.05)
+            sz.append(_sz_dev(m))
+            steps.append(list(m.evolve_config.local_steps))
+        runs[precision] = (np.array(sz), steps)
+    oo, so = [_sz_orc(st)], []
+    for _ in range(2):
+        st, ls = orc.tdvp_ps(st, w, 0.05)
+        oo.append(_sz_orc(st))
+        so.append(ls)
+    np.testing.assert_allclose(runs["fp64"][0], oo, atol=1e-8, rtol=0)
+    _assert_steps_match(runs["fp64"][1], so)
+    np.testing.assert_allclose(runs["mixed"][0], oo, atol=1e-4, rtol=0)
