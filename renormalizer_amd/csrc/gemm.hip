@@ -178,28 +178,45 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
     }
   };
 
-  auto gload = [&](int k0) {
+  // Per-thread element offsets, set for the first K step and advanced by
+  // kBK times the k stride after every load; a workgroup whose tile and K
+  // range lie wholly inside the matrices loads without bounds tests.
+  int64_t a_off[kPerThread], b_off[kPerThread];
+  int a_kk[kPerThread], b_kk[kPerThread];
+  bool a_in[kPerThread], b_in[kPerThread];
+#pragma unroll
+  for (int i = 0; i < kPerThread; ++i) {
+    const int e = tid + i * kThreads;
+    int mm, kk, nn;
+    a_coord(e, mm, kk);
+    a_in[i] = m0 + mm < g.M;
+    a_kk[i] = kk;
+    a_off[i] = a_in[i] ? aoff + (int64_t)(m0 + mm) * g.sAm + (int64_t)(kb + kk) * g.sAk : 0;
+    b_coord(e, kk, nn);
+    b_in[i] = n0 + nn < g.N;
+    b_kk[i] = kk;
+    b_off[i] = b_in[i] ? boff + (int64_t)(kb + kk) * g.sBk + (int64_t)(n0 + nn) * g.sBn : 0;
+  }
+  const int64_t a_step = (int64_t)kBK * g.sAk, b_step = (int64_t)kBK * g.sBk;
+  const bool full = m0 + kBM <= g.M && n0 + kBN <= g.N && (ke - kb) % kBK == 0;
+
+  // loads the K tile starting at k0 (absolute) and advances the offsets
+  auto gload = [&](int k0, auto checked) {
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i) {
-      const int e = tid + i * kThreads;
-      int mm, kk;
-      a_coord(e, mm, kk);
-      const int gm = m0 + mm, gk = k0 + kk;
       R re = (R)0, im = (R)0;
-      if (gm < g.M && gk < ke)
-        load_k<R, AKIND>(g.A, aoff + (int64_t)gm * g.sAm + (int64_t)gk * g.sAk, re, im);
+      if (!decltype(checked)::value || (a_in[i] && k0 + a_kk[i] < ke))
+        load_k<R, AKIND>(g.A, a_off[i], re, im);
       pa_re[i] = re;
       pa_im[i] = im;
-      int nn;
-      b_coord(e, kk, nn);
-      const int gn = n0 + nn;
-      const int gk2 = k0 + kk;
       re = (R)0;
       im = (R)0;
-      if (gn < g.N && gk2 < ke)
-        load_k<R, BKIND>(g.B, boff + (int64_t)gk2 * g.sBk + (int64_t)gn * g.sBn, re, im);
+      if (!decltype(checked)::value || (b_in[i] && k0 + b_kk[i] < ke))
+        load_k<R, BKIND>(g.B, b_off[i], re, im);
       pb_re[i] = re;
       pb_im[i] = im;
+      a_off[i] += a_step;
+      b_off[i] += b_step;
     }
   };
 
@@ -229,43 +246,49 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
   // one barrier per K step: the next tile goes to the other LDS buffer while
   // this one is multiplied (all waves left the other buffer at the previous
   // barrier)
-  gload(kb);
-  sstore(0);
-  __syncthreads();
-  int cur = 0;
-  for (int k0 = kb; k0 < ke; k0 += kBK) {
-    const bool more = k0 + kBK < ke;
-    if (more) gload(k0 + kBK);
+  auto run = [&](auto checked) {
+    gload(kb, checked);
+    sstore(0);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = kb; k0 < ke; k0 += kBK) {
+      const bool more = k0 + kBK < ke;
+      if (more) gload(k0 + kBK, checked);
 #pragma unroll
-    for (int ks = 0; ks < kBK / 4; ++ks) {
-      const int kr = ks * 4 + kq;
-      R a_re[2], a_im[2], b_re[2], b_im[2];
+      for (int ks = 0; ks < kBK / 4; ++ks) {
+        const int kr = ks * 4 + kq;
+        R a_re[2], a_im[2], b_re[2], b_im[2];
 #pragma unroll
-      for (int mb = 0; mb < 2; ++mb) {
-        a_re[mb] = sA[cur][0][kr][wm * 32 + mb * 16 + r16];
-        a_im[mb] = AREAL ? (R)0 : sA[cur][1][kr][wm * 32 + mb * 16 + r16];
-      }
-#pragma unroll
-      for (int nb = 0; nb < 2; ++nb) {
-        b_re[nb] = sB[cur][0][kr][wn * 32 + nb * 16 + r16];
-        b_im[nb] = sB[cur][1][kr][wn * 32 + nb * 16 + r16];
-      }
-#pragma unroll
-      for (int mb = 0; mb < 2; ++mb)
+        for (int mb = 0; mb < 2; ++mb) {
+          a_re[mb] = sA[cur][0][kr][wm * 32 + mb * 16 + r16];
+          a_im[mb] = AREAL ? (R)0 : sA[cur][1][kr][wm * 32 + mb * 16 + r16];
+        }
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
-          cr[mb][nb] = Mfma<R>::mac(a_re[mb], b_re[nb], cr[mb][nb]);
-          ci[mb][nb] = Mfma<R>::mac(a_re[mb], b_im[nb], ci[mb][nb]);
-          if (!AREAL) {
-            cr[mb][nb] = Mfma<R>::mac(-a_im[mb], b_im[nb], cr[mb][nb]);
-            ci[mb][nb] = Mfma<R>::mac(a_im[mb], b_re[nb], ci[mb][nb]);
-          }
+          b_re[nb] = sB[cur][0][kr][wn * 32 + nb * 16 + r16];
+          b_im[nb] = sB[cur][1][kr][wn * 32 + nb * 16 + r16];
         }
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+          for (int nb = 0; nb < 2; ++nb) {
+            cr[mb][nb] = Mfma<R>::mac(a_re[mb], b_re[nb], cr[mb][nb]);
+            ci[mb][nb] = Mfma<R>::mac(a_re[mb], b_im[nb], ci[mb][nb]);
+            if (!AREAL) {
+              cr[mb][nb] = Mfma<R>::mac(-a_im[mb], b_im[nb], cr[mb][nb]);
+              ci[mb][nb] = Mfma<R>::mac(a_im[mb], b_re[nb], ci[mb][nb]);
+            }
+          }
+      }
+      if (more) sstore(cur ^ 1);
+      __syncthreads();
+      cur ^= 1;
     }
-    if (more) sstore(cur ^ 1);
-    __syncthreads();
-    cur ^= 1;
-  }
+  };
+  if (full)
+    run(std::integral_constant<bool, false>());
+  else
+    run(std::integral_constant<bool, true>());
 
 #pragma unroll
   for (int mb = 0; mb < 2; ++mb)
