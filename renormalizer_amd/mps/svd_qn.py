@@ -43,26 +43,49 @@ def qn_blocks(qnbigl, qnbigr, qntot):
     return blocks
 
 
-def svd_qn_device(coef, qnbigl, qnbigr, qntot, system):
+_PLAN_CACHE = {}
+_PLAN_CACHE_MAX = 256
+
+
+def svd_qn_plan(qnbigl, qnbigr, qntot):
+    """Host part of svd_qn_device, independent of the coefficients: the
+    block index sets and the new bond quantum numbers, cached by the quantum
+    numbers (a sweep meets the same few patterns over and over)."""
+    qnbigl = np.ascontiguousarray(qnbigl)
+    qnbigr = np.ascontiguousarray(qnbigr)
+    key = (qnbigl.shape, qnbigl.tobytes(), qnbigr.shape, qnbigr.tobytes(), tuple(np.asarray(qntot).tolist()))
+    plan = _PLAN_CACHE.get(key)
+    if plan is None:
+        rows = int(np.prod(qnbigl.shape[:-1]))
+        cols = int(np.prod(qnbigr.shape[:-1]))
+        blocks = qn_blocks(qnbigl, qnbigr, qntot)
+        if not blocks:
+            raise ValueError("Invalid quantum number")
+        qnl, qnr = [], []
+        for nl, nr, lset, rset in blocks:
+            k = min(len(lset), len(rset))
+            qnl += [nl] * k
+            qnr += [nr] * k
+        plan = (rows, cols, [(b[2], b[3]) for b in blocks], qnl, qnr)
+        if len(_PLAN_CACHE) >= _PLAN_CACHE_MAX:
+            _PLAN_CACHE.pop(next(iter(_PLAN_CACHE)))
+        _PLAN_CACHE[key] = plan
+    return plan
+
+
+def svd_qn_device(coef, qnbigl, qnbigr, qntot, system, plan=None):
     """QR ("L") / RQ ("R") of the device coefficient tensor.
 
     Returns (u, qnl, vt, qnr): u (rows x k), vt (k x cols) complex128 device
     tensors with coef = u @ vt, and the new bond quantum numbers as lists of
-    tuples (the reference's ``new_qnl``/``new_qnr``).
+    tuples (the reference's ``new_qnl``/``new_qnr``).  ``plan`` is the
+    result of svd_qn_plan for these quantum numbers, if already at hand.
     """
-    rows = int(np.prod(qnbigl.shape[:-1]))
-    cols = int(np.prod(qnbigr.shape[:-1]))
-    coef2 = coef.reshape(rows, cols)
-    blocks = qn_blocks(qnbigl, qnbigr, qntot)
-    if not blocks:
-        raise ValueError("Invalid quantum number")
-    u, vt = native.block_qr(system, coef2, [(b[2], b[3]) for b in blocks], v_transposed=True)
-    qnl, qnr = [], []
-    for nl, nr, lset, rset in blocks:
-        k = min(len(lset), len(rset))
-        qnl += [nl] * k
-        qnr += [nr] * k
-    return u, qnl, vt, qnr
+    if plan is None:
+        plan = svd_qn_plan(qnbigl, qnbigr, qntot)
+    rows, cols, blocks, qnl, qnr = plan
+    u, vt = native.block_qr(system, coef.reshape(rows, cols), blocks, v_transposed=True)
+    return u, list(qnl), vt, list(qnr)
 
 
 def optimized_svd(a, full_matrices):
