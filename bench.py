@@ -147,6 +147,10 @@ def roofline(prof, seconds, steps):
         "frac": round(achieved / peak, 4), "traffic": measured_traffic(name), "kernel": name,
         "launches": p["executed"], "launches_queued": p["queued"], "launches_timed": p["timed"],
         "avg_launch_us": round(p["timed_ms"] * 1e3 / p["timed"], 2),
+        # event time of sampled launches cancelled on the device: the
+        # bracket's own share (queue latency) of avg_launch_us
+        "event_overhead_us": round(p["timed_cancelled_ms"] * 1e3 / p["timed_cancelled"], 2)
+        if p["timed_cancelled"] else None,
         "flop_per_launch": p["timed_flop"] / p["timed"],
         "gemm_flop_per_step": total_flop / steps,
         "gemm_time_fraction": round(total_ms * 1e-3 / seconds, 3),

@@ -189,6 +189,11 @@ int reno_zgemm(int M, int N, int K, const void* A, const void* B, void* C,
 int reno_prof_enable(int period);
 int reno_prof_collect(int nkinds, double* ms, double* flop_timed, int* timed, double* flop,
                       int* executed, int* queued);
+/* The event-timed launches of the sample that were cancelled on the device
+ * (they return at once), per instantiation: count and summed event
+ * milliseconds -- the part of a measured launch that is the event bracket's
+ * own (queue latency), not the kernel. */
+int reno_prof_cancelled(int nkinds, double* ms, int* count);
 /* Number of GEMM kernel instantiations; for 0 <= kind < that number, fills
  * info[0..5] = precision, A k-contiguous, B k-contiguous, and the storage
  * kinds of A, B, C (0 complex128, 1 complex64, 2 float64, 3 float32). */

@@ -730,6 +730,28 @@ int prof_collect(int nkinds, double* ms, double* flop_timed, int* timed, double*
   return RENO_OK;
 }
 
+// Per instantiation, the event-timed launches that were cancelled on the
+// device (they return at once): their count and summed event milliseconds,
+// i.e. what the event bracket itself adds to a launch.  Waits for the device.
+int prof_cancelled(int nkinds, double* ms, int* count) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  for (int k = 0; k < nkinds; ++k) {
+    ms[k] = 0.0;
+    count[k] = 0;
+  }
+  const int st = harvest_markers();
+  if (st != RENO_OK) return st;
+  for (auto& r : g_prof) {
+    if (r.kind >= nkinds || r.ran || !r.a) continue;
+    RENO_HIP_CHECK(hipEventSynchronize(r.b));
+    float t = 0.f;
+    RENO_HIP_CHECK(hipEventElapsedTime(&t, r.a, r.b));
+    ms[r.kind] += t;
+    count[r.kind] += 1;
+  }
+  return RENO_OK;
+}
+
 // ---- lane-layout probe: one MFMA 16x16x4 on row-major a (16x4), b (4x16);
 // writes the raw accumulator registers c[lane*4 + r] -------------------------
 template <typename R>
@@ -771,6 +793,14 @@ extern "C" int reno_prof_kind_info(int kind, int* info) {
     info[5] = k.ckind;
   }
   return reno::kNumGemmKinds;
+}
+
+extern "C" int reno_prof_cancelled(int nkinds, double* ms, int* count) {
+  if (nkinds < 0 || (nkinds > 0 && (!ms || !count))) {
+    reno::set_error("prof_cancelled: bad arguments");
+    return RENO_EINVAL;
+  }
+  return reno::prof_cancelled(nkinds, ms, count);
 }
 
 extern "C" int reno_prof_collect(int nkinds, double* ms, double* flop_timed, int* timed,

@@ -34,6 +34,7 @@ EXPORTS = (
     "reno_hop_apply", "reno_env_update", "reno_env_close", "reno_expm_krylov",
     "reno_expm_krylov_async", "reno_block_qr", "reno_heff_diag", "reno_heff_dense", "reno_zgemm",
     "reno_probe_mfma16", "reno_prof_enable", "reno_prof_collect", "reno_prof_kind_info",
+    "reno_prof_cancelled",
 )
 
 # demangled names of the GEMM kernel instantiations, indexed like
@@ -124,6 +125,8 @@ def load_library(path: str = LIB_PATH):
         lib.reno_prof_enable.argtypes = [c_int]
         lib.reno_prof_kind_info.restype = c_int
         lib.reno_prof_kind_info.argtypes = [c_int, ctypes.POINTER(c_int)]
+        lib.reno_prof_cancelled.restype = c_int
+        lib.reno_prof_cancelled.argtypes = [c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_int)]
         lib.reno_prof_collect.restype = c_int
         lib.reno_prof_collect.argtypes = [c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_double),
                                           ctypes.POINTER(c_int), ctypes.POINTER(c_double),
@@ -475,7 +478,9 @@ def prof_collect():
     """{kernel name: {"timed", "timed_ms", "timed_flop": executed launches of
     the event-timed sample, their GPU ms and algorithmic FLOP; "executed",
     "flop": all executed launches and their FLOP; "queued": all launches,
-    including those cancelled on the device}}."""
+    including those cancelled on the device; "timed_cancelled",
+    "timed_cancelled_ms": event-timed launches that were cancelled on the
+    device and their summed event time}}."""
     names = gemm_kind_names()
     n = len(names)
     ms = (ctypes.c_double * n)()
@@ -485,8 +490,12 @@ def prof_collect():
     ex = (ctypes.c_int * n)()
     qd = (ctypes.c_int * n)()
     check(lib().reno_prof_collect(n, ms, flt, tm, fl, ex, qd), "reno_prof_collect")
+    cms = (ctypes.c_double * n)()
+    cct = (ctypes.c_int * n)()
+    check(lib().reno_prof_cancelled(n, cms, cct), "reno_prof_cancelled")
     return {names[k]: {"timed": tm[k], "timed_ms": ms[k], "timed_flop": flt[k], "executed": ex[k],
-                       "flop": fl[k], "queued": qd[k]}
+                       "flop": fl[k], "queued": qd[k], "timed_cancelled": cct[k],
+                       "timed_cancelled_ms": cms[k]}
             for k in range(n) if qd[k]}
 
 
