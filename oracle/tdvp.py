@@ -346,6 +346,149 @@ def tdvp_ps(state: State, mpo, evolve_dt):
 
 
 # ---------------------------------------------------------------------------
+# two-site TDVP-PS with SVD truncation
+
+
+def merge_w(w0, w1):
+    """Two MPO sites as one site of physical dimension d0*d1:
+    W[b, (d g), (e h), j] = sum_f W0[b,d,e,f] W1[f,g,h,j].  hop1 with it is
+    the two-site hop_expr "abc,bdef,fghj,ljk,cehk->adgl" (hop_expr.py:99-103)
+    with the (d, g) and (e, h) pairs fused."""
+    m = np.einsum("bdef,fghj->bdgehj", w0, w1)
+    return m.reshape(w0.shape[0], w0.shape[1] * w1.shape[1], w0.shape[2] * w1.shape[2], w1.shape[3])
+
+
+def select_basis(vset, sset, qnlist, compset, mmax, percent=0):
+    """reference mps/lib.py:253-322: keep the mmax largest weights (a
+    percent share first taken equally from every quantum-number block);
+    returns (ms, mpsdim, mpsqn, compms) with compms columns scaled by the
+    weights."""
+    qnlist = [tuple(q) for q in qnlist]
+    qnset = set(qnlist)
+    basdic = {i: [qnlist[i], sset[i]] for i in range(len(qnlist)) if qnlist[i] in qnset}
+
+    def block_select(qn, cnt):
+        blk = sorted(((i, v) for i, v in basdic.items() if v[0] == qn), key=lambda x: x[1][1], reverse=True)
+        got = [i for i, _ in blk[:min(cnt, len(blk))]]
+        for i in got:
+            del basdic[i]
+        return got
+
+    nbasis = min(len(basdic), mmax)
+    sidx = []
+    if percent != 0:
+        per = int(nbasis * percent / len(qnset))
+        for qn in qnset:
+            sidx += block_select(qn, per)
+    rest = sorted(basdic.items(), key=lambda x: x[1][1], reverse=True)
+    sidx += [i for i, _ in rest[:nbasis - len(sidx)]]
+    ms = np.zeros((vset.shape[0], len(sidx)), dtype=vset.dtype)
+    comp = None if compset is None else np.zeros((compset.shape[0], len(sidx)), dtype=compset.dtype)
+    mpsqn = []
+    for k, i in enumerate(sidx):
+        ms[:, k] = vset[:, i]
+        if comp is not None and i < compset.shape[1]:
+            comp[:, k] = compset[:, i] * sset[i]
+        mpsqn.append(qnlist[i])
+    return ms, len(sidx), np.array(mpsqn), comp
+
+
+def m_trunc(sigma, bond, max_dim=None, threshold=None):
+    """CompressConfig.compute_m_trunc (utils/configs.py:196-219): fixed
+    (max_dim at the bond), threshold (normalised weights above it), or the
+    smaller of the two."""
+    cands = []
+    if max_dim is not None:
+        cands.append(min(int(max_dim), len(sigma)))
+    if threshold is not None:
+        cands.append(int(np.sum(sigma / np.linalg.norm(sigma) > threshold)))
+    return min(cands)
+
+
+def tdvp_ps2(state: State, mpo, evolve_dt, max_dim=None, threshold=None):
+    """One two-site TDVP-PS time step, reference mps/mps.py:1406-1517 with
+    ivp_solver="krylov" and real ``evolve_dt``: per bond the two-site
+    Krylov step (-i dt/2), SVD truncation (mp.py:651-780, single root,
+    percent 0), the environment update, and the one-site step (+i dt/2) of
+    the site left behind followed by a QR moving the centre on (mp.py:
+    890-908); then normalize("mps_only").  Returns (new_state,
+    local_steps).  Pinned by: exactness vs the dense propagator when no
+    truncation happens, the 1-site oracle's closed forms, and the
+    truncation against a direct SVD of the dense two-site tensor."""
+    mps = state.copy()
+    n = len(mps)
+    environ = Environ(mps.sites, mpo)
+    local_steps = []
+    for _ in range(2):
+        for imps in mps.iter_idx_list(full=False):
+            if mps.to_right:
+                lidx, c0, c1, ridx = imps - 1, imps, imps + 1, imps + 2
+                c2, last = c1, n - 2
+            else:
+                lidx, c0, c1, ridx = imps - 2, imps - 1, imps, imps + 1
+                c2, last = c0, 1
+            l_array = environ.read("L", lidx)
+            r_array = environ.read("R", ridx)
+            ms2 = np.tensordot(mps.sites[c0], mps.sites[c1], axes=1)
+            shape2 = ms2.shape
+            w2 = merge_w(mpo[c0], mpo[c1])
+            fused = (shape2[0], shape2[1] * shape2[2], shape2[3])
+            mps_t, j = expm_krylov(lambda y: hop1(l_array, w2, r_array, y.reshape(fused)).ravel(),
+                                   -1j * evolve_dt / 2, ms2.ravel())
+            local_steps.append(j)
+            # big quantum numbers of the two-site block (mp.py:308-352)
+            qnl = np.array(mps.qn[c0])
+            qnr = np.array(mps.qn[c1 + 1])
+            qnbigl = add_outer(qnl, mps.sigmaqn[c0])
+            qnbigr = add_outer(mps.sigmaqn[c1], qnr)
+            system = "L" if mps.to_right else "R"
+            u, su, qnlnew, v, sv, qnrnew = svd_qn(mps_t.reshape(shape2), qnbigl, qnbigr, mps.qntot,
+                                                  system=system)
+            if mps.to_right:
+                mt = m_trunc(su, c0 + 1, max_dim, threshold)
+                ms, msdim, msqn, comp = select_basis(u, su, qnlnew, v, mt)
+                mps.sites[c0] = ms.reshape(list(qnbigl.shape[:-1]) + [msdim])
+                mps.sites[c1] = np.moveaxis(comp.reshape(list(qnbigr.shape[:-1]) + [msdim]), -1, 0)
+                mps.qnidx = c1
+            else:
+                mt = m_trunc(sv, c1, max_dim, threshold)
+                ms, msdim, msqn, comp = select_basis(v, sv, qnrnew, u, mt)
+                mps.sites[c1] = np.moveaxis(ms.reshape(list(qnbigr.shape[:-1]) + [msdim]), -1, 0)
+                mps.sites[c0] = comp.reshape(list(qnbigl.shape[:-1]) + [msdim])
+                mps.qnidx = c0
+            mps.qn[c1] = np.array(msqn).reshape(msdim, -1)
+            if imps == last:
+                continue
+            if mps.to_right:
+                l_array = environ.system("L", lidx + 1, mps.sites, mpo, l_array)
+            else:
+                r_array = environ.system("R", ridx - 1, mps.sites, mpo, r_array)
+            shape1 = mps.sites[c2].shape
+            w = mpo[c2]
+            mps_t, j = expm_krylov(lambda y: hop1(l_array, w, r_array, y.reshape(shape1)).ravel(),
+                                   1j * evolve_dt / 2, mps.sites[c2].ravel())
+            local_steps.append(j)
+            mps.sites[c2] = mps_t.reshape(shape1)
+            # _push_cano(c2): QR moving the centre to the next site
+            qnbigl1, qnbigr1 = mps.get_big_qn(c2)
+            uq, qnlset, vq, qnrset = svd_qn(mps.sites[c2], qnbigl1, qnbigr1, mps.qntot, QR=True,
+                                            system=system, full_matrices=False)
+            if mps.to_right:
+                mps.sites[c2 + 1] = np.tensordot(vq.T, mps.sites[c2 + 1], axes=1)
+                mps.sites[c2] = uq.reshape(list(shape1[:-1]) + [uq.shape[1]])
+                mps.qn[c2 + 1] = np.array(qnlset).reshape(len(qnlset), -1)
+                mps.qnidx = c2 + 1
+            else:
+                mps.sites[c2 - 1] = np.tensordot(mps.sites[c2 - 1], uq, axes=1)
+                mps.sites[c2] = vq.T.reshape([vq.shape[1]] + list(shape1[1:]))
+                mps.qn[c2] = np.array(qnrset).reshape(len(qnrset), -1)
+                mps.qnidx = c2 - 1
+        mps.switch_direction()
+    normalize_mps_only(mps)
+    return mps, local_steps
+
+
+# ---------------------------------------------------------------------------
 # norms and observables
 
 

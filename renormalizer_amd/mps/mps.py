@@ -80,6 +80,16 @@ def adaptive_tdvp(fun):
     return adaptive_fun
 
 
+
+def _describe(local_steps):
+    """scipy.stats.describe of the Krylov matvec counts (reference
+    mps.py:1400-1402); equal counts make its skewness undefined, which scipy
+    reports as a precision-loss warning."""
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        return stats.describe(local_steps)
+
 class Mps(MatrixProduct):
     def __init__(self):
         super().__init__()
@@ -330,6 +340,7 @@ class Mps(MatrixProduct):
     def evolve(self, mpo, evolve_dt, normalize=True) -> "Mps":
         method = {
             EvolveMethod.tdvp_ps: self._evolve_tdvp_ps,
+            EvolveMethod.tdvp_ps2: self._evolve_tdvp_ps2,
         }.get(self.evolve_config.method)
         if method is None:
             raise NotImplementedError(
@@ -369,9 +380,90 @@ class Mps(MatrixProduct):
             mps, environ, counts = self._tdvp_ps_sweep(mpo, evolve_dt, prec, None, None)
             synchronous = True
         local_steps = [c for _, c in counts]
-        mps.evolve_config.stat = stats.describe(local_steps)
+        mps.evolve_config.stat = _describe(local_steps)
         mps.evolve_config.local_steps = local_steps
         mps._env_cache = (mpo, prec, environ, (complex(evolve_dt), dict(counts), synchronous))
+        return mps
+
+    @adaptive_tdvp
+    def _evolve_tdvp_ps2(self, mpo: Mpo, evolve_dt) -> "Mps":
+        """Two-site TDVP projector splitting with SVD truncation, two
+        half-sweeps (reference mps.py:1406-1517), on the device: per bond the
+        two-site Krylov step (-i dt/2) with the merged two-site MPO, the
+        truncation by the blocked SVD and basis selection of the compress
+        config (mp.py:651-780, host LAPACK as in the reference), the
+        environment update, and the one-site step (+i dt/2) of the site left
+        behind with the device QR that moves the centre on (mp.py:890-908).
+        The bond dimension can grow up to the compress config's limit."""
+        if self.evolve_config.ivp_solver != "krylov":
+            raise NotImplementedError("only the Krylov propagator is on the device path")
+        from renormalizer_amd.mps.gs import _merge_w, _update_mps
+        prec = _PREC[self.evolve_config.precision]
+        mps = self.copy() if np.iscomplex(evolve_dt) else self.to_complex()
+        if not mps.on_device:
+            mps.to_device()
+        dev = mps._mp[0].device
+        wt = mpo.device_tensors(prec, dev)
+        rdt = torch.float64 if prec == native.PREC_FP64 else torch.float32
+        n = len(mps)
+        environ = Environ(mps._mp, wt, None, prec=prec)
+        w2_cache = {}
+        local_steps = []
+        for _ in range(2):
+            for imps in mps.iter_idx_list(full=False):
+                if mps.to_right:
+                    lidx, c0, c1, ridx = imps - 1, imps, imps + 1, imps + 2
+                    c2, last = c1, n - 2
+                else:
+                    lidx, c0, c1, ridx = imps - 2, imps - 1, imps, imps + 1
+                    c2, last = c0, 1
+                l_array = environ.read("L", lidx)
+                r_array = environ.read("R", ridx)
+                ms2 = torch.tensordot(mps._mp[c0], mps._mp[c1], dims=1)
+                shape2 = list(ms2.shape)
+                if (c0, c1) not in w2_cache:
+                    w2 = _merge_w(np.asarray(mpo[c0], dtype=np.float64), np.asarray(mpo[c1], dtype=np.float64))
+                    w2_cache[(c0, c1)] = torch.from_numpy(w2).to(device=dev, dtype=rdt).contiguous()
+                fused = ms2.reshape(shape2[0], shape2[1] * shape2[2], shape2[3]).contiguous()
+                mps_t, j = native.expm_krylov(l_array, w2_cache[(c0, c1)], r_array, fused,
+                                              -1j * evolve_dt / 2, prec)
+                local_steps.append(j)
+                qnbigl, qnbigr, _ = mps._get_big_qn([c0, c1], with_qnmat=False)
+                _update_mps(mps, mps_t.reshape(shape2), [c0, c1], qnbigl, qnbigr, 0)
+                if imps == last:
+                    continue
+                if mps.to_right:
+                    l_array = environ.GetLR("L", lidx + 1, mps._mp, wt, itensor=l_array, method="System")
+                else:
+                    r_array = environ.GetLR("R", ridx - 1, mps._mp, wt, itensor=r_array, method="System")
+                site = mps._mp[c2].contiguous()
+                shape1 = list(site.shape)
+                mps_t, j = native.expm_krylov(l_array, wt[c2], r_array, site, 1j * evolve_dt / 2, prec)
+                local_steps.append(j)
+                # move the canonical centre on (_push_cano) with the device QR
+                qnbigl, qnbigr, _ = mps._get_big_qn([c2], with_qnmat=False)
+                system = "L" if mps.to_right else "R"
+                u, qnlset, vt, qnrset = svd_qn_device(mps_t, qnbigl, qnbigr, mps.qntot, system)
+                if mps.to_right:
+                    kt = u.shape[1]
+                    mps._mp[c2] = u.reshape(shape1[:-1] + [kt])
+                    nxt = mps._mp[c2 + 1]
+                    mps._mp[c2 + 1] = native.zgemm(vt, nxt.reshape(nxt.shape[0], -1)).reshape(
+                        [kt] + list(nxt.shape[1:]))
+                    mps.qn[c2 + 1] = np.array(qnlset, dtype=int).reshape(kt, -1)
+                    mps.qnidx = c2 + 1
+                else:
+                    kt = vt.shape[0]
+                    mps._mp[c2] = vt.reshape([kt] + shape1[1:])
+                    prev = mps._mp[c2 - 1]
+                    mps._mp[c2 - 1] = native.zgemm(prev.reshape(-1, prev.shape[-1]), u).reshape(
+                        list(prev.shape[:-1]) + [kt])
+                    mps.qn[c2] = np.array(qnrset, dtype=int).reshape(kt, -1)
+                    mps.qnidx = c2 - 1
+            mps._switch_direction()
+        mps._invalidate()
+        mps.evolve_config.stat = _describe(local_steps)
+        mps.evolve_config.local_steps = local_steps
         return mps
 
     # number of time steps redone with host waits because an asynchronous
@@ -545,11 +637,19 @@ class Mps(MatrixProduct):
 
     # ---- on-disk format (reference mp.py:1085-1113, mps.py:352-386) -------
     def dump(self, fname):
+        """Protocol "0.4" (reference mp.py:1085-1113): ``mt_<i>`` site
+        arrays, ``subqn_<i>`` bond quantum numbers, ``qn`` (the same as an
+        object array, which the reference loader reads), ``qnidx``,
+        ``qntot``, ``to_right``, ``coeff``, ``version``, ``nsites``."""
         data = {"version": "0.4", "nsites": len(self)}
         for i, mt in enumerate(self.numpy_sites()):
             data[f"mt_{i}"] = mt
-        for i, q in enumerate(self.qn):
-            data[f"subqn_{i}"] = np.array(q)
+        qn = [np.array(q) for q in self.qn]
+        arr = np.empty(len(qn), object)
+        arr[:] = qn
+        data["qn"] = arr
+        for i, q in enumerate(qn):
+            data[f"subqn_{i}"] = q
         data["qnidx"] = self.qnidx
         data["qntot"] = np.array(self.qntot)
         data["to_right"] = self.to_right
@@ -558,18 +658,31 @@ class Mps(MatrixProduct):
 
     @classmethod
     def load(cls, model: Model, fname: str):
-        f = np.load(fname)
-        mps = cls()
-        mps.model = model
-        n = int(f["nsites"])
-        mps._mp = [f[f"mt_{i}"] for i in range(n)]
-        mps.dtype = np.complex128 if any(np.iscomplexobj(m) for m in mps._mp) else np.float64
-        mps.qn = [f[f"subqn_{i}"].astype(int).reshape(-1, model.qn_size) for i in range(n + 1)]
-        mps.qnidx = int(f["qnidx"])
-        mps.qntot = f["qntot"].astype(int)
-        mps.to_right = bool(f["to_right"])
-        if "coeff" in f:
-            mps.coeff = f["coeff"].item()
+        """Reads protocols "0.1", "0.3" and "0.4" (reference mps.py:352-386)
+        without unpickling: the bond quantum numbers come from the
+        ``subqn_<i>`` arrays the reference writes beside its pickled ``qn``.
+        Protocol "0.2" keeps its coefficient in a pickled array and is
+        refused, as is an unknown version."""
+        with np.load(fname, allow_pickle=False) as f:
+            version = str(f["version"])
+            n = int(f["nsites"])
+            mps = cls()
+            mps.model = model
+            mps._mp = [f[f"mt_{i}"] for i in range(n)]
+            mps.dtype = np.complex128 if any(np.iscomplexobj(m) for m in mps._mp) else np.float64
+            mps.qn = [f[f"subqn_{i}"].astype(int).reshape(-1, model.qn_size) for i in range(n + 1)]
+            mps.qnidx = int(f["qnidx"])
+            mps.qntot = f["qntot"].astype(int)
+            if version == "0.1":
+                mps.to_right = bool(f["left"])
+                mps.coeff = 1
+            elif version in ("0.3", "0.4"):
+                mps.to_right = bool(f["to_right"])
+                mps.coeff = f["coeff"].item(0)
+            elif version == "0.2":
+                raise ValueError("dump protocol 0.2 stores its coefficient pickled; not read")
+            else:
+                raise ValueError(f"Unknown dump version: {version}")
         return mps
 
     def __str__(self):
