@@ -6,13 +6,16 @@
 //
 // R phase, per panel of NB = 8 columns: (a) a small launch applies the
 // previous panel's block reflector to the new panel's columns (one workgroup
-// per column); (b) k_qr_step: workgroup 0 factors the panel with its columns
-// held in registers (rows r0 + tid + 256 i), one reduction per column (norm of
-// x, the dot products with the later panel columns; the pivot row goes
-// through LDS), and forms the panel's compact-WY factor T (zlarft, forward
-// columnwise); the other workgroups meanwhile apply the previous panel's
-// reflector to one trailing column each.  Reductions use DPP lane moves
-// (VALU) rather than LDS shuffles; barriers order LDS only.
+// per column, column and reflector rows held in registers); (b) k_qr_step:
+// workgroup 0 factors the panel with its columns held in registers (rows
+// r0 + tid + 512 i), one reduction per column (norm of x, the dot products
+// with the later panel columns; the pivot row goes through LDS), and forms
+// the panel's compact-WY factor T (zlarft, forward columnwise); the other
+// workgroups meanwhile apply the previous panel's reflector to one trailing
+// column each.  512 threads per workgroup keep every register array in
+// VGPRs (two waves per SIMD).  A reduction adds quad partners by DPP, passes
+// the quad sums through LDS and finishes each value in one 16-lane row;
+// barriers order LDS only.
 //
 // Q phase (zungqr): with the explicit V (m x k, unit lower trapezoidal) and
 // the full k x k factor T of H_0...H_{k-1} = I - V T V^H,
@@ -27,8 +30,9 @@
 namespace reno {
 
 constexpr int kQrNB = 8;
-constexpr int kPanelThreads = 256;  // one wave per SIMD: up to 512 registers per lane
-constexpr int kMaxRpt = 10;         // rows per thread: m <= 2560
+constexpr int kPanelThreads = 512;  // two waves per SIMD: up to 256 registers per lane
+constexpr int kQuads = kPanelThreads / 4;
+constexpr int kMaxRpt = 5;          // rows per thread: m <= 2560
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -48,117 +52,122 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Sum of a double over the 64 lanes of a wavefront with DPP lane moves (VALU
-// only; no LDS crossbar traffic): quad xor 1 and 2, half-row mirror, row
-// mirror, then row_bcast:15 / row_bcast:31 fold the four rows into lane 63,
-// whose value is broadcast.  Fixed order: deterministic.
-template <int CTRL, int ROWMASK>
+// A double moved between lanes by a DPP pattern that reads a valid lane for
+// every active lane (quad permutations, row mirrors).
+template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double wave_sum(double v) {
-  v += dpp_f64<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
-  v += dpp_f64<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
-  v += dpp_f64<0x141, 0xF>(v);  // row_half_mirror
-  v += dpp_f64<0x140, 0xF>(v);  // row_mirror
-  v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
-  v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 
-// Sum NV doubles across the workgroup; results in out[0..NV).
-// sh holds (kPanelThreads/64) * NV doubles.  Fixed order: deterministic.
+// Sum NV doubles over the workgroup in a fixed order (deterministic);
+// results in out[0..NV), visible to every thread on return.  Stage 1: quad
+// sums by two DPP exchanges (quad_perm [1,0,3,2], [2,3,0,1]), stored by the
+// first lane of each quad (sh holds NV * kQuads doubles).  Stage 2: one
+// 16-lane row per value; lane p adds quad sums p, p + 16, ... (consecutive
+// lanes read consecutive words), then the row is summed by the two quad
+// exchanges and the half-row and row mirrors.  Every lane of a row ends
+// with the same bits (each exchange adds the same two operands).
 template <int NV>
-__device__ __forceinline__ void block_reduce(double (&vals)[NV], double* sh, double* out) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  constexpr int NW = kPanelThreads / 64;
+__device__ __forceinline__ void lds_reduce(double (&vals)[NV], double* sh, double* out) {
+  static_assert(16 * NV <= kPanelThreads, "one 16-lane row per value");
+  const int tid = threadIdx.x;
+  const bool lead = (tid & 3) == 0;
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
-    const double v = wave_sum(vals[i]);
-    if (lane == 0) sh[wave * NV + i] = v;
+    double v = vals[i];
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    if (lead) sh[i * kQuads + (tid >> 2)] = v;
   }
   lds_barrier();
-  for (int i = threadIdx.x; i < NV; i += kPanelThreads) {
+  if (tid < 16 * NV) {
+    const int v = tid >> 4, p = tid & 15;
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) s += sh[w * NV + i];
-    out[i] = s;
-  }
-  lds_barrier();
-}
-
-// g = F (sum over the workgroup of w) for the 8x8 factor F (LDS): the
-// reduction, then 8 threads form g in LDS (keeps F out of the registers of
-// every thread).
-__device__ __forceinline__ void reduce_times_F(double (&w)[2 * kQrNB], const double2* F, double* sh,
-                                               double* red, double2* sG) {
-  block_reduce<2 * kQrNB>(w, sh, red);
-  if (threadIdx.x < kQrNB) {
-    const int q = threadIdx.x;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int p = 0; p < kQrNB; ++p)
-      acc = cadd(acc, cmul(F[q * kQrNB + p], make_double2(red[2 * p], red[2 * p + 1])));
-    sG[q] = acc;
+    for (int k = 0; k < kQuads / 16; ++k) s += sh[v * kQuads + k * 16 + p];
+    s += dpp_f64<0xB1>(s);
+    s += dpp_f64<0x4E>(s);
+    s += dpp_f64<0x141>(s);
+    s += dpp_f64<0x140>(s);
+    if (p == 0) out[v] = s;
   }
   lds_barrier();
 }
 
 // Column c (rows jp..m-1) <- (I + V F V^H) column, V = Vall[:, jp:jp+jbp],
-// F the row-major 8x8 factor (-T^H of panel jp) in LDS.  Rolled row loops:
-// V is streamed twice from L2, the registers stay small.
-__device__ void apply_prev_column(double2* __restrict__ X, int m, int ld, int c, int jp, int jbp,
+// F the row-major 8x8 factor (-T^H of panel jp) in LDS.  The column and the
+// V rows jp + tid + 512 i are loaded once (all loads in flight together) and
+// both passes run from registers.
+template <int RPT>
+__device__ __forceinline__ void apply_prev_column(double2* __restrict__ X, int m, int ld, int c, int jp, int jbp,
                                   const double2* __restrict__ Vall, const double2* F,
                                   double* sh, double* red, double2* sG) {
   const int tid = threadIdx.x;
+  const double2 zero = make_double2(0.0, 0.0);
+  double2 x[RPT];
+  double2 v[RPT][kQrNB];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    const int r = jp + tid + kPanelThreads * i;
+    const bool in = r < m;
+    x[i] = in ? X[r + (int64_t)c * ld] : zero;
+#pragma unroll
+    for (int q = 0; q < kQrNB; ++q) v[i][q] = (in && q < jbp) ? Vall[r + (int64_t)(jp + q) * m] : zero;
+  }
   double w[2 * kQrNB];
 #pragma unroll
   for (int q = 0; q < 2 * kQrNB; ++q) w[q] = 0.0;
-#pragma unroll 2
-  for (int r = jp + tid; r < m; r += kPanelThreads) {
-    const double2 x = X[r + (int64_t)c * ld];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
 #pragma unroll
     for (int q = 0; q < kQrNB; ++q) {
-      const double2 v = q < jbp ? Vall[r + (int64_t)(jp + q) * m] : make_double2(0.0, 0.0);
-      const double2 p = cmulc(v, x);
+      const double2 p = cmulc(v[i][q], x[i]);
       w[2 * q] += p.x;
       w[2 * q + 1] += p.y;
     }
   }
-  reduce_times_F(w, F, sh, red, sG);
+  lds_reduce<2 * kQrNB>(w, sh, red);
+  if (tid < kQrNB) {
+    const int q = tid;
+    double2 acc = zero;
+    for (int p = 0; p < kQrNB; ++p) acc = cadd(acc, cmul(F[q * kQrNB + p], make_double2(red[2 * p], red[2 * p + 1])));
+    sG[q] = acc;
+  }
+  lds_barrier();
   double2 g[kQrNB];
 #pragma unroll
   for (int q = 0; q < kQrNB; ++q) g[q] = sG[q];
-#pragma unroll 2
-  for (int r = jp + tid; r < m; r += kPanelThreads) {
-    double2 y = X[r + (int64_t)c * ld];
 #pragma unroll
-    for (int q = 0; q < kQrNB; ++q) {
-      const double2 v = q < jbp ? Vall[r + (int64_t)(jp + q) * m] : make_double2(0.0, 0.0);
-      y = cadd(y, cmul(v, g[q]));
-    }
-    X[r + (int64_t)c * ld] = y;
+  for (int i = 0; i < RPT; ++i) {
+    const int r = jp + tid + kPanelThreads * i;
+    double2 y = x[i];
+#pragma unroll
+    for (int q = 0; q < kQrNB; ++q) y = cadd(y, cmul(v[i][q], g[q]));
+    if (r < m) X[r + (int64_t)c * ld] = y;
   }
 }
 
 // The panel columns, one runtime loop iteration per column jj (the code is
 // kept small: a fully unrolled panel overflows the instruction cache).  The
-// registers hold the not-yet-factored columns rotated so that column jj is
-// always slot 0 (slot c = column jj + c); after each step the slots shift
-// left by one and the new reflector v_jj enters slot 7, so every register
-// index is a compile-time constant and, at step jj, v_q (q < jj) sits in
-// slot 8 - jj + q.  One reduction per column gives |x|^2, the dot products
+// registers hold rows r0 + tid + 512 i (r0 <= j0, rows from r0 to j0 - 1
+// lie above the panel and pass through unchanged) of the not-yet-factored
+// columns rotated so that column jj is always slot 0 (slot c = column
+// jj + c); after each step the slots shift left by one and the new
+// reflector v_jj enters slot 7, so every register index is a compile-time
+// constant and, at step jj, v_q (q < jj) sits in slot 8 - jj + q.  One
+// reduction per column gives |x|^2, the dot products
 // g_c = sum_{r>j} conj(a_rj) a_rc with the later columns, and the same sums
 // with the earlier reflectors, from which column jj of V^H V and hence
 // column jj of the compact-WY factor T (zlarft, forward columnwise) follow
 // without another pass over V.  The pivot row (alpha, a_jc and the v_q(j))
 // goes through LDS (double-buffered by column parity).  Each column is
-// written to X / Vall as soon as it is final.
+// written to X / Vall as soon as it is final.  Rows of chunk i >= 1 lie
+// below every panel column (r >= r0 + 512 > j) and, for i < RPT - 1, inside
+// the matrix; rows past m hold zeros throughout.
 template <int RPT>
-__device__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, int m, int ld, int j0, int jb,
+__device__ __forceinline__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, int m, int ld, int j0, int jb,
                              double2* __restrict__ X, double2* __restrict__ Vall, double* sh,
                              double* red, double2 (*piv)[kQrNB], double2* sTau,
                              double2 (*sT)[kQrNB]) {
@@ -174,7 +183,7 @@ __device__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, int m, int ld, in
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
       const int r = r0 + tid + kPanelThreads * i;
-      if (r > j && r < m) {
+      if (i > 0 || r > j) {
         const double2 x = a[i][0];
         vals[0] += x.x * x.x + x.y * x.y;
 #pragma unroll
@@ -188,7 +197,7 @@ __device__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, int m, int ld, in
         for (int c = 0; c < kQrNB; ++c) pv[c] = a[i][c];
       }
     }
-    block_reduce<1 + 2 * kQrNB>(vals, sh, red);
+    lds_reduce<1 + 2 * kQrNB>(vals, sh, red);
     const double xnorm2 = red[0];
     const double2 alpha = pv[0];
     double2 tj = make_double2(0.0, 0.0);
@@ -224,7 +233,8 @@ __device__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, int m, int ld, in
       sT[jj][jj] = tj;
     }
     const bool refl = !(tj.x == 0.0 && tj.y == 0.0);
-    // f_c = conj(tau) (a_jc + conj(scale) g_c): coefficient of v in H^H a_c
+    // f_c = conj(tau) (a_jc + conj(scale) g_c): coefficient of v in H^H a_c.
+    // Without a reflection scale is 1 and every f_c is zero.
     double2 f[kQrNB];
     const double2 ct = make_double2(tj.x, -tj.y);
     const double2 cs = make_double2(scale.x, -scale.y);
@@ -237,51 +247,64 @@ __device__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, int m, int ld, in
     for (int i = 0; i < RPT; ++i) {
       const int r = r0 + tid + kPanelThreads * i;
       const double2 x = a[i][0];
-      double2 vr = (refl && r > j) ? cmul(x, scale) : x;  // v_r below the diagonal
-      double2 out = vr;                                   // final column value
-      if (r == j) {
-        out = make_double2(beta, 0.0);
-        vr = make_double2(1.0, 0.0);
-      }
-      if (r < j) out = x;
-      if (r < m) {
-        X[r + (int64_t)j * ld] = out;
-        Vall[r + (int64_t)j * m] = r < j ? make_double2(0.0, 0.0) : vr;
-      }
-      // later columns: a_c -= v f_c on rows >= j, then shift left (f_c is
-      // zero for the reflector slots); v_jj enters the last slot
+      if (i == 0) {
+        double2 vr = r > j ? cmul(x, scale) : x;  // v_r below the diagonal
+        double2 out = vr;                         // final column value
+        if (r == j) {
+          out = make_double2(beta, 0.0);
+          vr = make_double2(1.0, 0.0);
+        }
+        if (r < j) out = x;
+        if (r < m) {
+          X[r + (int64_t)j * ld] = out;
+          Vall[r + (int64_t)j * m] = r < j ? make_double2(0.0, 0.0) : vr;
+        }
+        // later columns: a_c -= v f_c on rows >= j, then shift left (f_c is
+        // zero for the reflector slots); v_jj enters the last slot
 #pragma unroll
-      for (int c = 1; c < kQrNB; ++c) {
-        double2 y = a[i][c];
-        if (r >= j) y = csub(y, cmul(vr, f[c]));
-        a[i][c - 1] = y;
+        for (int c = 1; c < kQrNB; ++c) {
+          double2 y = a[i][c];
+          if (r >= j) y = csub(y, cmul(vr, f[c]));
+          a[i][c - 1] = y;
+        }
+        a[i][kQrNB - 1] = (r < j || r >= m) ? make_double2(0.0, 0.0) : vr;
+      } else {
+        const double2 vr = cmul(x, scale);  // zero for rows past m
+        if (i < RPT - 1 || r < m) {
+          X[r + (int64_t)j * ld] = vr;
+          Vall[r + (int64_t)j * m] = vr;
+        }
+#pragma unroll
+        for (int c = 1; c < kQrNB; ++c) a[i][c - 1] = csub(a[i][c], cmul(vr, f[c]));
+        a[i][kQrNB - 1] = vr;
       }
-      a[i][kQrNB - 1] = (r < j || r >= m) ? make_double2(0.0, 0.0) : vr;
     }
   }
 }
 
-// The panel columns into registers (rows j0 + tid + 256 i, column c in slot c).
+// The panel columns into registers (rows r0 + tid + 512 i, column j0 + c in
+// slot c; zeros past m and past the panel).
 template <int RPT>
 __device__ __forceinline__ void load_panel(double2 (&a)[RPT][kQrNB], const double2* __restrict__ X,
-                                           int m, int ld, int j0, int jb) {
+                                           int m, int ld, int r0, int j0, int jb) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int i = 0; i < RPT; ++i) {
-    const int r = j0 + tid + kPanelThreads * i;
+    const int r = r0 + tid + kPanelThreads * i;
 #pragma unroll
     for (int c = 0; c < kQrNB; ++c)
       a[i][c] = (r < m && c < jb) ? X[r + (int64_t)(j0 + c) * ld] : make_double2(0.0, 0.0);
   }
 }
 
-// Apply the block reflector of panel jp (-T^H in LDS) to columns
-// c_begin + blockIdx.x (one workgroup per column).
+// Apply the block reflector of panel jp (-T^H in Tall) to columns
+// c_begin + blockIdx.x (one workgroup per column, rows jp + tid + 512 i).
+template <int RPT>
 __global__ __launch_bounds__(kPanelThreads) void k_qr_apply(double2* __restrict__ X, int m, int ld,
                                                             int jp, int jbp, int c_begin, int c_end,
                                                             const double2* __restrict__ Vall,
                                                             const double2* __restrict__ Tall) {
-  __shared__ double sh[(kPanelThreads / 64) * 2 * kQrNB];
+  __shared__ double sh[2 * kQrNB * kQuads];
   __shared__ double red[2 * kQrNB];
   __shared__ double2 sF[kQrNB * kQrNB];
   __shared__ double2 sG[kQrNB];
@@ -289,12 +312,13 @@ __global__ __launch_bounds__(kPanelThreads) void k_qr_apply(double2* __restrict_
   if (c >= c_end) return;
   if (threadIdx.x < kQrNB * kQrNB) sF[threadIdx.x] = Tall[(size_t)(jp / kQrNB) * 2 * kQrNB * kQrNB + threadIdx.x];
   lds_barrier();
-  apply_prev_column(X, m, ld, c, jp, jbp, Vall, sF, sh, red, sG);
+  apply_prev_column<RPT>(X, m, ld, c, jp, jbp, Vall, sF, sh, red, sG);
 }
 
 // One step of the R phase: workgroup 0 factors panel [j0, j0+jb) (already
 // updated by all previous reflectors); workgroups 1.. apply the reflector of
-// panel jp (jp < 0: none) to columns j0+jb, j0+jb+1, ..., n-1.
+// panel jp (jp < 0: none) to columns j0+jb, j0+jb+1, ..., n-1.  Both hold
+// rows from r0 = jp (j0 for the first panel), RPT per thread.
 // Outputs: R / v in X, tau[j0..), explicit V columns in Vall (zeros above
 // the unit diagonal) and Tall[p*128 ..] = -T^H (row-major 8x8) followed by -T.
 template <int RPT>
@@ -303,8 +327,8 @@ __global__ __launch_bounds__(kPanelThreads) void k_qr_step(double2* __restrict__
                                                            double2* __restrict__ tau,
                                                            double2* __restrict__ Vall,
                                                            double2* __restrict__ Tall) {
-  __shared__ double sh[(kPanelThreads / 64) * 64];
-  __shared__ double red[64];
+  __shared__ double sh[(1 + 2 * kQrNB) * kQuads];
+  __shared__ double red[1 + 2 * kQrNB];
   __shared__ double2 sT[kQrNB][kQrNB];
   __shared__ double2 sTau[kQrNB];
   __shared__ double2 sPiv[2][kQrNB];
@@ -316,15 +340,16 @@ __global__ __launch_bounds__(kPanelThreads) void k_qr_step(double2* __restrict__
     if (c >= n || jp < 0) return;
     if (tid < kQrNB * kQrNB) sF[tid] = Tall[(size_t)(jp / kQrNB) * 2 * kQrNB * kQrNB + tid];
     lds_barrier();
-    apply_prev_column(X, m, ld, c, jp, jbp, Vall, sF, sh, red, sG);
+    apply_prev_column<RPT>(X, m, ld, c, jp, jbp, Vall, sF, sh, red, sG);
     return;
   }
   if (jb == 0) return;
+  const int r0 = jp >= 0 ? jp : j0;
   for (int t = tid; t < kQrNB * kQrNB; t += kPanelThreads) sT[t / kQrNB][t % kQrNB] = make_double2(0.0, 0.0);
   double2 a[RPT][kQrNB];
-  load_panel<RPT>(a, X, m, ld, j0, jb);
-  factor_panel<RPT>(a, j0, m, ld, j0, jb, X, Vall, sh, red, sPiv, sTau, sT);
-  for (int r = tid; r < j0; r += kPanelThreads)
+  load_panel<RPT>(a, X, m, ld, r0, j0, jb);
+  factor_panel<RPT>(a, r0, m, ld, j0, jb, X, Vall, sh, red, sPiv, sTau, sT);
+  for (int r = tid; r < r0; r += kPanelThreads)
     for (int c = 0; c < jb; ++c) Vall[r + (int64_t)(j0 + c) * m] = make_double2(0.0, 0.0);
   lds_barrier();
   double2* Tb = Tall + (size_t)(j0 / kQrNB) * 2 * kQrNB * kQrNB;
@@ -405,6 +430,9 @@ bool qr_supported(int m, int n) {
 template <int RPT>
 static void launch_step(hipStream_t s, int grid, double2* X, int m, int n, int jp, int jbp, int j0,
                         int jb, double2* tau, double2* Vall, double2* Tall) {
+  if (jp >= 0 && jb > 0)
+    hipLaunchKernelGGL(k_qr_apply<RPT>, dim3(jb), dim3(kPanelThreads), 0, s, X, m, m, jp, jbp, j0, j0 + jb,
+                       (const double2*)Vall, (const double2*)Tall);
   hipLaunchKernelGGL(k_qr_step<RPT>, dim3(grid), dim3(kPanelThreads), 0, s, X, m, m, n, jp, jbp, j0,
                      jb, tau, Vall, Tall);
 }
@@ -449,24 +477,17 @@ int qr_householder(hipStream_t s, int m, int n, double2* X, double2* Q, void* ws
     const int jbp = ip > 0 ? std::min(kQrNB, k - jp) : 0;
     const int trail = ip > 0 ? std::max(0, n - (j0 + jb)) : 0;
     if (jb == 0 && trail == 0) break;
-    if (jp >= 0 && jb > 0) {
-      hipLaunchKernelGGL(k_qr_apply, dim3(jb), dim3(kPanelThreads), 0, s, X, m, m, jp, jbp, j0, j0 + jb,
-                         (const double2*)Vall, (const double2*)Tall);
-      RENO_CHECK_LAUNCH();
-    }
-    const int rpt = std::max(1, (m - j0 + kPanelThreads - 1) / kPanelThreads);
+    // rows from r0 = jp (j0 for the first panel): those the previous
+    // reflector and the panel touch
+    const int r0 = jp >= 0 ? jp : j0;
+    const int rpt = std::max(1, (m - r0 + kPanelThreads - 1) / kPanelThreads);
     const int grid = 1 + trail;
     switch (rpt) {
       case 1: launch_step<1>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
       case 2: launch_step<2>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
       case 3: launch_step<3>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
       case 4: launch_step<4>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
-      case 5: launch_step<5>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
-      case 6: launch_step<6>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
-      case 7: launch_step<7>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
-      case 8: launch_step<8>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
-      case 9: launch_step<9>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
-      default: launch_step<10>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
+      default: launch_step<5>(s, grid, X, m, n, jp, jbp, j0, jb, tau, Vall, Tall); break;
     }
     RENO_CHECK_LAUNCH();
   }
