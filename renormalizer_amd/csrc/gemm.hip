@@ -8,9 +8,10 @@
 // steps of 16.  Operands are staged through LDS with real and imaginary parts
 // split into separate planes (conjugation applied on the way in), so that
 // each wave feeds v_mfma_{f64,f32}_16x16x4 directly from LDS: a wave owns a
-// 32x32 sub-tile = 2x2 MFMA blocks, and a complex multiply-accumulate is four
-// real MFMAs (two when A is real, e.g. the MPO stage).  The next K-tile is
-// fetched into registers while the current one is multiplied.
+// 32x32 sub-tile = 2x2 MFMA blocks.  An FP64 complex multiply-accumulate is
+// three real MFMAs (Gauss: Ar Br, Ai Bi, (Ar + Ai)(Br + Bi)), the FP32 one
+// four, and two when A is real (the MPO stage).  The next K-tile is fetched
+// into registers while the current one is multiplied.
 //
 // MFMA 16x16x4 lane layout (checked on the device by reno_probe_mfma16):
 //   A operand: lane l holds A[l % 16][l / 16]
@@ -145,7 +146,11 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
   const int ke = min(g.K, kb + g.kchunk);
   const int64_t coff = bat * g.sCb + (int64_t)blockIdx.z * gridDim.y * g.sCb;
 
-  acc_t cr[2][2], ci[2][2];
+  // FP64 complex x complex products use three real products per block
+  // (Gauss): P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); then
+  // Re = P1 - P2, Im = P3 - P1 - P2.  cr holds P1, ci holds P3 and cq P2.
+  constexpr bool K3M = std::is_same<R, double>::value && !AREAL;
+  acc_t cr[2][2], ci[2][2], cq[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -154,6 +159,7 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
       for (int r = 0; r < 4; ++r) {
         cr[i][j][r] = (R)0;
         ci[i][j][r] = (R)0;
+        cq[i][j][r] = (R)0;
       }
     }
 
@@ -268,17 +274,33 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
           b_re[nb] = sB[cur][0][kr][wn * 32 + nb * 16 + r16];
           b_im[nb] = sB[cur][1][kr][wn * 32 + nb * 16 + r16];
         }
+        if constexpr (K3M) {
+          R a_s[2], b_s[2];
 #pragma unroll
-        for (int mb = 0; mb < 2; ++mb)
+          for (int mb = 0; mb < 2; ++mb) a_s[mb] = a_re[mb] + a_im[mb];
 #pragma unroll
-          for (int nb = 0; nb < 2; ++nb) {
-            cr[mb][nb] = Mfma<R>::mac(a_re[mb], b_re[nb], cr[mb][nb]);
-            ci[mb][nb] = Mfma<R>::mac(a_re[mb], b_im[nb], ci[mb][nb]);
-            if (!AREAL) {
-              cr[mb][nb] = Mfma<R>::mac(-a_im[mb], b_im[nb], cr[mb][nb]);
-              ci[mb][nb] = Mfma<R>::mac(a_im[mb], b_re[nb], ci[mb][nb]);
+          for (int nb = 0; nb < 2; ++nb) b_s[nb] = b_re[nb] + b_im[nb];
+#pragma unroll
+          for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+              cr[mb][nb] = Mfma<R>::mac(a_re[mb], b_re[nb], cr[mb][nb]);
+              cq[mb][nb] = Mfma<R>::mac(a_im[mb], b_im[nb], cq[mb][nb]);
+              ci[mb][nb] = Mfma<R>::mac(a_s[mb], b_s[nb], ci[mb][nb]);
             }
-          }
+        } else {
+#pragma unroll
+          for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+              cr[mb][nb] = Mfma<R>::mac(a_re[mb], b_re[nb], cr[mb][nb]);
+              ci[mb][nb] = Mfma<R>::mac(a_re[mb], b_im[nb], ci[mb][nb]);
+              if (!AREAL) {
+                cr[mb][nb] = Mfma<R>::mac(-a_im[mb], b_im[nb], cr[mb][nb]);
+                ci[mb][nb] = Mfma<R>::mac(a_im[mb], b_re[nb], ci[mb][nb]);
+              }
+            }
+        }
       }
       if (more) sstore(cur ^ 1);
       __syncthreads();
@@ -298,9 +320,13 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm * 32 + mb * 16 + Mfma<R>::out_row(kq, r);
         const int col = n0 + wn * 32 + nb * 16 + r16;
+        R re = cr[mb][nb][r], im = ci[mb][nb][r];
+        if constexpr (K3M) {
+          re = cr[mb][nb][r] - cq[mb][nb][r];
+          im = ci[mb][nb][r] - cr[mb][nb][r] - cq[mb][nb][r];
+        }
         if (row < g.M && col < g.N)
-          store_k<R, CKIND>(g.C, coff + (int64_t)row * g.sCm + (int64_t)col * g.sCn,
-                            cr[mb][nb][r], ci[mb][nb][r], g.beta);
+          store_k<R, CKIND>(g.C, coff + (int64_t)row * g.sCm + (int64_t)col * g.sCn, re, im, g.beta);
       }
 }
 
