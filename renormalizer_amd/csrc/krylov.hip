@@ -20,7 +20,11 @@
 //     per chunk (normally once per call);
 //   * reno_expm_krylov_async queues up to max_matvec iterations plus the
 //     finalisation and returns at once; the outcome lands in a device info
-//     array that the caller reads whenever it likes.
+//     array that the caller reads whenever it likes.  Its convergence checks
+//     (the small exponential and the combination) run on a second stream:
+//     the next matvec needs only the recurrence, so it overlaps them.  The
+//     state block then records the iteration of each kind of exit, and the
+//     finalisation takes the one the reference's loop would reach first.
 #include <math.h>
 #include <string.h>
 
@@ -43,8 +47,32 @@ enum {
   KS_RESIDX = 3,   // result buffer holding the converged vector (status 1)
   KS_NMATVEC = 4,  // matvecs used (reference j + 1)
   KS_COUNT = 5,    // blocks finished in the current launch (last-block folds)
+  KS_BITER = 6,    // reference iteration j of a breakdown / full-space exit (else INT_MAX)
+  KS_CONV = 7,     // 1 + iteration j of the converged allclose check (else 0)
   KS_INTS = 8
 };
+
+// How the call ended, from the state block: 1 converged (result in
+// res[RESIDX], CONV matvecs), 2 breakdown / 3 full space (combination of
+// MFINAL vectors, NMATVEC matvecs), 0 iteration budget exhausted, -1 zero v0.
+// The checks may run on a second stream, behind the recurrence, so a check
+// can converge at an iteration after the one where the recurrence already
+// stopped; the earlier event wins, as in the reference's loop (a breakdown
+// and a converged check at the same iteration give the same vector, and the
+// breakdown is taken).
+__host__ __device__ __forceinline__ int final_kind(const int* st) {
+  const int status = st[KS_STATUS];
+  if (status < 0) return -1;
+  const int conv = st[KS_CONV];
+  if (conv && conv - 1 < st[KS_BITER]) return 1;
+  return status;
+}
+
+// A check at iteration j is void once a check converged, v0 was zero, or the
+// recurrence stopped at or before j.
+__device__ __forceinline__ bool check_void(const int* st, int j) {
+  return st[KS_CONV] != 0 || st[KS_STATUS] < 0 || st[KS_BITER] <= j;
+}
 
 // one block per 256 elements up to kMaxPartials blocks: the bond-matrix
 // vectors (65 536 elements at M = 256) still spread over 256 CUs
@@ -97,10 +125,11 @@ __device__ __forceinline__ bool fold_beta(const double* __restrict__ norm_part, 
     beta[j - 1] = be;
     sig[j] = 1.0 / be;
     if (be < thr) {
-      st[KS_STOP] = 1;
       st[KS_STATUS] = 2;
       st[KS_MFINAL] = j;
       st[KS_NMATVEC] = j;
+      st[KS_BITER] = j - 1;
+      st[KS_STOP] = 1;
     }
   }
   *be_out = be;
@@ -161,7 +190,7 @@ __global__ __launch_bounds__(kRedThreads) void k_kry_init(const double* __restri
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += kRedThreads) acc += partial[i];
   const double s = block_sum(acc, sh);
-  if (threadIdx.x < KS_INTS) st[threadIdx.x] = 0;
+  if (threadIdx.x < KS_INTS) st[threadIdx.x] = threadIdx.x == KS_BITER ? 0x7fffffff : 0;
   __syncthreads();
   if (threadIdx.x == 0) {
     const double nrm = sqrt(s);
@@ -197,10 +226,11 @@ __global__ __launch_bounds__(kRedThreads) void k_fold_alpha(const double* __rest
 // Krylov space equals the full space after iteration n-1 (krylov.py:59-61)
 __global__ void k_mark_full(int n, int* st) {
   if (st[KS_STOP]) return;
-  st[KS_STOP] = 1;
   st[KS_STATUS] = 3;
   st[KS_MFINAL] = n;
   st[KS_NMATVEC] = n;
+  st[KS_BITER] = n - 1;
+  st[KS_STOP] = 1;
 }
 
 // dst = src / (*den)
@@ -277,8 +307,11 @@ __global__ __launch_bounds__(64) void k_tri_coef(const double* __restrict__ alph
                                                  const double* __restrict__ scal, double dt_re,
                                                  double dt_im, int m_in, int final,
                                                  const int* st, double2* __restrict__ coef) {
-  if (!final && st[KS_STOP]) return;
-  if (final && st[KS_STATUS] != 2 && st[KS_STATUS] != 3) return;
+  if (!final && check_void(st, m_in - 1)) return;
+  if (final) {
+    const int kind = final_kind(st);
+    if (kind != 2 && kind != 3) return;
+  }
   __shared__ double sJ[kChebMax];
   __shared__ int sK;
   const int m = final ? st[KS_MFINAL] : m_in;
@@ -422,7 +455,7 @@ __global__ __launch_bounds__(kRedThreads) void k_combine(const double2* __restri
                                                          const double2* __restrict__ res_old,
                                                          double2* __restrict__ res_new, int j,
                                                          int residx, int* st) {
-  if (st[KS_STOP]) return;
+  if (check_void(st, j)) return;
   __shared__ double sh[kRedThreads];
   __shared__ double2 sc[64];
   for (int k = threadIdx.x; k < nv && k < 64; k += kRedThreads) {
@@ -467,10 +500,9 @@ __global__ __launch_bounds__(kRedThreads) void k_combine(const double2* __restri
     if ((tot & ((1 << 12) - 1)) == (int)gridDim.x) {
       st[KS_COUNT] = 0;
       if ((tot >> 12) == 0) {
-        st[KS_STOP] = 1;
-        st[KS_STATUS] = 1;
         st[KS_RESIDX] = residx;
-        st[KS_NMATVEC] = j + 1;
+        st[KS_CONV] = j + 1;
+        st[KS_STOP] = 1;
       }
     }
   }
@@ -487,7 +519,7 @@ __global__ __launch_bounds__(kRedThreads) void k_final_out(const double2* __rest
                                                            const double2* __restrict__ res1,
                                                            double2* __restrict__ out,
                                                            const int* st) {
-  const int status = st[KS_STATUS];
+  const int status = final_kind(st);
   if (status == 1) {
     const double2* src = st[KS_RESIDX] ? res1 : res0;
     for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
@@ -512,9 +544,9 @@ __global__ __launch_bounds__(kRedThreads) void k_final_out(const double2* __rest
 // info[0] = matvecs used; info[1] = 1 finished by the reference's rule,
 // 0 iteration budget exhausted (out not written), -1 zero starting vector
 __global__ void k_info(const int* st, int queued, int* info) {
-  const int status = st[KS_STATUS];
-  info[0] = status == 0 ? queued : st[KS_NMATVEC];
-  info[1] = status == 0 ? 0 : (status < 0 ? -1 : 1);
+  const int kind = final_kind(st);
+  info[0] = kind == 0 ? queued : (kind == 1 ? st[KS_CONV] : st[KS_NMATVEC]);
+  info[1] = kind == 0 ? 0 : (kind < 0 ? -1 : 1);
 }
 
 struct KrylovLayout {
@@ -559,10 +591,41 @@ struct HostStage {
 };
 static thread_local HostStage t_stage;
 
+// The convergence checks of the asynchronous entry point run on a second
+// stream of the same device (default priority, non-blocking), so the next
+// matvec does not wait for them; one event hands each check its recurrence,
+// another hands the finalisation the last check.  Created on first use, per
+// thread and device, and kept until the process ends.
+struct CheckStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fwd = nullptr, back = nullptr;
+};
+constexpr int kMaxDevices = 64;
+static thread_local CheckStream t_check[kMaxDevices];
+
+static int check_stream(CheckStream** out) {
+  int dev = 0;
+  RENO_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("expm_krylov_async: device %d beyond the supported %d", dev, kMaxDevices);
+    return RENO_EINVAL;
+  }
+  CheckStream& c = t_check[dev];
+  if (!c.s) {
+    RENO_HIP_CHECK(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
+    RENO_HIP_CHECK(hipEventCreateWithFlags(&c.fwd, hipEventDisableTiming));
+    RENO_HIP_CHECK(hipEventCreateWithFlags(&c.back, hipEventDisableTiming));
+  }
+  *out = &c;
+  return RENO_OK;
+}
+
 // One call's device buffers and launch parameters.
 struct KrylovRun {
   const reno_hop_desc* op;
   hipStream_t s;
+  hipStream_t cs;              // stream of the convergence checks (s, or a CheckStream)
+  hipEvent_t ev_fwd, ev_back;  // only with a separate check stream
   int64_t n;
   int kcap;
   KrylovLayout L;
@@ -593,6 +656,8 @@ static void bind_run(KrylovRun& r, const reno_hop_desc* op, hipStream_t s, void*
                      double dt_re, double dt_im) {
   r.op = op;
   r.s = s;
+  r.cs = s;
+  r.ev_fwd = r.ev_back = nullptr;
   r.n = hop_vec_len(op);
   r.kcap = kcap;
   r.L = krylov_layout(op, kcap);
@@ -635,10 +700,12 @@ static int queue_start(KrylovRun& r, const void* v0) {
   return RENO_OK;
 }
 
-// k_tri_coef instantiation for up to m entries
+// k_tri_coef instantiation for up to m entries (checks on the check stream,
+// the final one on the main stream)
 static void launch_tri_coef(KrylovRun& r, int m, int final) {
+  hipStream_t st = final ? r.s : r.cs;
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(1), dim3(64), 0, r.s, (const double*)r.d_alpha,
+    hipLaunchKernelGGL(kern, dim3(1), dim3(64), 0, st, (const double*)r.d_alpha,
                        (const double*)r.d_beta, (const double*)r.scal, r.dt_re, r.dt_im, m, final,
                        (const int*)r.st, r.d_coef);
   };
@@ -681,8 +748,12 @@ static int queue_iter(KrylovRun& r, int64_t j) {
     const int m = (int)j + 1;
     const int dst = (int)((j / 2) & 1);
     const bool compare = j > 4;
+    if (r.cs != r.s) {
+      RENO_HIP_CHECK(hipEventRecord(r.ev_fwd, r.s));
+      RENO_HIP_CHECK(hipStreamWaitEvent(r.cs, r.ev_fwd, 0));
+    }
     launch_tri_coef(r, m, 0);
-    hipLaunchKernelGGL(k_combine, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.V, r.n,
+    hipLaunchKernelGGL(k_combine, dim3(r.np), dim3(kRedThreads), 0, r.cs, (const double2*)r.V, r.n,
                        (int64_t)r.vstride, m, (const double2*)r.d_coef, (const double*)r.d_sig,
                        compare ? (const double2*)r.res[dst ^ 1] : (const double2*)nullptr,
                        r.res[dst], (int)j, dst, r.st);
@@ -693,6 +764,10 @@ static int queue_iter(KrylovRun& r, int64_t j) {
 
 // final combination / copy into out; m_max bounds MFINAL
 static int queue_final(KrylovRun& r, void* out, int m_max) {
+  if (r.cs != r.s) {
+    RENO_HIP_CHECK(hipEventRecord(r.ev_back, r.cs));
+    RENO_HIP_CHECK(hipStreamWaitEvent(r.s, r.ev_back, 0));
+  }
   launch_tri_coef(r, m_max, 1);
   hipLaunchKernelGGL(k_final_out, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.V,
                      r.n, (int64_t)r.vstride, (const double2*)r.d_coef, (const double*)r.d_sig,
@@ -758,13 +833,13 @@ extern "C" int reno_expm_krylov(const reno_hop_desc* op, double dt_re, double dt
                                   hipMemcpyDeviceToHost, s));
     RENO_HIP_CHECK(hipEventRecord(t_stage.ev, s));
     RENO_HIP_CHECK(hipEventSynchronize(t_stage.ev));
-    const int status = t_stage.pin[KS_STATUS];
-    if (status < 0) {
+    const int kind = final_kind(t_stage.pin);
+    if (kind < 0) {
       set_error("expm_krylov: zero starting vector (krylov.py:44 assert)");
       return RENO_EINVAL;
     }
-    if (status > 0) {
-      *n_matvec = t_stage.pin[KS_NMATVEC];
+    if (kind > 0) {
+      *n_matvec = kind == 1 ? t_stage.pin[KS_CONV] : t_stage.pin[KS_NMATVEC];
       return queue_final(r, out, (int)jend);
     }
     if (j >= jmax) {
@@ -794,6 +869,11 @@ extern "C" int reno_expm_krylov_async(const reno_hop_desc* op, double dt_re, dou
   }
   KrylovRun r;
   bind_run(r, op, s, ws, kcap, dt_re, dt_im);
+  CheckStream* cs = nullptr;
+  RENO_TRY(check_stream(&cs));
+  r.cs = cs->s;
+  r.ev_fwd = cs->fwd;
+  r.ev_back = cs->back;
   RENO_TRY(queue_start(r, v0));
   for (int64_t j = 0; j < jmax; ++j) RENO_TRY(queue_iter(r, j));
   RENO_TRY(queue_final(r, out, (int)jmax));
