@@ -27,6 +27,8 @@ def contract_one_site(environ, ms, mo, domain, ms_conj=None):
     assert domain in ("L", "R")
     if ms_conj is None:
         ms_conj = ms.conj()
+    if ms.ndim == 4:
+        return _contract_one_site_ancilla(environ, ms, mo, domain, ms_conj)
     if domain == "L":
         assert environ.shape[0] == ms_conj.shape[0]
         assert environ.shape[1] == mo.shape[0]
@@ -44,10 +46,34 @@ def contract_one_site(environ, ms, mo, domain, ms_conj=None):
     return t
 
 
+def _contract_one_site_ancilla(environ, ms, mo, domain, ms_conj):
+    """Environment update for a 4-leg density-operator site (M_l, d, d_a,
+    M_r), reference mps/lib.py:206-211 (L) and 238-243 (R): the ancilla
+    index l is shared by bra and ket and untouched by the operator.
+    Pinned by: thermal expectation values vs the dense Gibbs state."""
+    if domain == "L":
+        t = np.tensordot(environ, ms_conj, axes=([0], [0]))       # abc,adlf->bcdlf
+        t = np.tensordot(t, mo, axes=([0, 2], [0, 1]))             # bcdlf,bdeg->clfeg
+        t = np.tensordot(t, ms, axes=([0, 3, 1], [0, 1, 2]))       # clfeg,celh->fgh
+        return t
+    t = np.tensordot(ms_conj, environ, axes=([3], [0]))            # fdla,abc->fdlbc
+    t = np.tensordot(t, mo, axes=([1, 3], [1, 3]))                 # fdlbc,gdeb->flcge
+    t = np.tensordot(t, ms, axes=([4, 1, 2], [1, 2, 3]))           # flcge,helc->fgh
+    return t
+
+
 def hop1(ltensor, mo, rtensor, c):
     """1-site effective Hamiltonian matvec "abc, bdef, lfk, cek -> adl"
-    (reference mps/hop_expr.py:68-79).  Pinned by: dense H_eff assembled from
-    the Kronecker product of L, W, R on tiny shapes; Hermiticity."""
+    (reference mps/hop_expr.py:68-79); for a 4-leg density-operator site
+    "abc, bdef, lfk, cegk -> adgl" (hop_expr.py:80-91), the ancilla g a
+    spectator.  Pinned by: dense H_eff assembled from the Kronecker product
+    of L, W, R on tiny shapes; Hermiticity; thermal expectation values vs
+    the dense Gibbs state (4-leg)."""
+    if c.ndim == 4:
+        t = np.tensordot(ltensor, c, axes=([2], [0]))              # abc,cegk->abegk
+        t = np.tensordot(t, mo, axes=([1, 2], [0, 2]))             # abegk,bdef->agkdf
+        t = np.tensordot(t, rtensor, axes=([2, 4], [2, 1]))        # agkdf,lfk->agdl
+        return t.transpose(0, 2, 1, 3)                             # -> adgl
     t = np.tensordot(ltensor, c, axes=([2], [0]))                  # abc,cek->abek
     t = np.tensordot(t, mo, axes=([1, 2], [0, 2]))                 # abek,bdef->akdf
     t = np.tensordot(t, rtensor, axes=([1, 3], [2, 1]))            # akdf,lfk->adl
@@ -497,7 +523,10 @@ def dot(sites_a, sites_b):
     e0 = np.eye(1, 1)
     for mt1, mt2 in zip(sites_a, sites_b):
         e0 = np.tensordot(e0, mt2, 1)
-        e0 = np.tensordot(e0, mt1, ([0, 1], [0, 1])).T
+        if mt1.ndim == 4:   # density-operator site: system and ancilla summed
+            e0 = np.tensordot(e0, mt1, ([0, 1, 2], [0, 1, 2])).T
+        else:
+            e0 = np.tensordot(e0, mt1, ([0, 1], [0, 1])).T
     return complex(e0[0, 0])
 
 
@@ -537,11 +566,16 @@ def calc_1site_rdm(state, idx):
 
 
 def expectation(state, mpo):
-    """<psi|O|psi>, reference mps.py:471-525."""
+    """<psi|O|psi>, reference mps.py:471-525; for 4-leg density-operator
+    sites the ancilla index is traced between bra and ket (mpdm.py:99-122)."""
     conj = [s.conj() for s in state.sites]
     env = Environ(state.sites, mpo, "R", conj)
     r = env.read("R", 1) if len(state) > 1 else np.ones((1, 1, 1))
-    val = np.einsum("abc,cfh,bdfg,ade,egh->", np.ones((1, 1, 1)), state.sites[0], mpo[0], conj[0], r)
+    if state.sites[0].ndim == 4:
+        val = np.einsum("abc,cflh,bdfg,adle,egh->", np.ones((1, 1, 1)), state.sites[0], mpo[0],
+                        conj[0], r)
+    else:
+        val = np.einsum("abc,cfh,bdfg,ade,egh->", np.ones((1, 1, 1)), state.sites[0], mpo[0], conj[0], r)
     if np.isclose(float(val.imag), 0):
         return float(val.real)
     return complex(val)

@@ -19,4 +19,5 @@ from renormalizer_amd.utils import (Quantity, CompressCriteria, CompressConfig, 
 from renormalizer_amd.model import (Op, OpSum, BasisSet, BasisSHO, BasisHalfSpin,  # noqa: E402
                                     BasisSimpleElectron, Phonon, Mol, Model, HolsteinModel,
                                     SpinBosonModel)
-from renormalizer_amd.mps import Mpo, Mps, construct_mps_mpo, optimize_mps  # noqa: E402
+from renormalizer_amd.mps import (Mpo, Mps, MpDm, ThermalProp, construct_mps_mpo,  # noqa: E402
+                                  optimize_mps)

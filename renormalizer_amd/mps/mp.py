@@ -98,7 +98,7 @@ class MatrixProduct:
     def bond_dims_exact(self) -> np.ndarray:
         """Bond dimensions of an exact factorisation (reference mp.py:130-142)."""
         p = np.array(self.pbond_dims, dtype=float)
-        if self.is_mpo:
+        if self.is_mpo or getattr(self, "is_mpdm", False):
             p = p ** 2
         with np.errstate(over="ignore"):
             d1 = [1] + list(np.cumprod(p))
@@ -218,7 +218,7 @@ class MatrixProduct:
         new.compress_config.update(self.compress_config)
         dt = new.dtype
         n = self.site_num
-        if self.is_mps:
+        if self.is_mps or getattr(self, "is_mpdm", False):
             new._mp[0] = np.concatenate([self[0], other[0]], axis=2).astype(dt)
             for i in range(1, n - 1):
                 a, b = self[i], other[i]

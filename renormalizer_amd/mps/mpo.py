@@ -117,8 +117,15 @@ class Mpo(MatrixProduct):
         new = mp.copy()
         if np.iscomplexobj(self._mp[0]) and not new.is_complex:
             new.to_complex(inplace=True)
-        if mp.is_mps:
-            for i, (w, m) in enumerate(zip(self._mp, mp._mp)):
+        ws = self._mp
+        if getattr(mp, "is_mpdm", False):
+            # a purified density operator keeps system and ancilla merged in
+            # one index: the operator acts on the system part (reference
+            # mpo.py:363-378 contracts it with the first index)
+            from renormalizer_amd.mps.mpdm import ancilla_mpo
+            ws = ancilla_mpo(self)._mp
+        if mp.is_mps or getattr(mp, "is_mpdm", False):
+            for i, (w, m) in enumerate(zip(ws, mp._mp)):
                 assert w.shape[2] == m.shape[1]
                 t = np.moveaxis(np.tensordot(w, m, axes=([2], [1])), 3, 1)
                 new._mp[i] = np.ascontiguousarray(

@@ -635,6 +635,28 @@ class Mps(MatrixProduct):
             rdm[i] = native.env_close(x, r).reshape(d, d).cpu().numpy()
         return rdm
 
+    def calc_bond_singular_values(self) -> np.ndarray:
+        """Singular values on every bond (reference mps.py:1759-1773): a host
+        copy made right-canonical, then one left-to-right SVD sweep without
+        truncation; rows padded with zeros."""
+        mps = self.copy().to_host()
+        mps.ensure_right_canonical()
+        _, s_array = mps.compress(temp_m_trunc=np.iinfo(np.int32).max, ret_s=True)
+        return s_array
+
+    def calc_bond_entropy(self, s_array: np.ndarray = None) -> np.ndarray:
+        """Von Neumann entropy -sum p ln p, p = s^2 / sum s^2, on every bond
+        (reference mps.py:1775-1793, utils.py:41-48)."""
+        if s_array is None:
+            s_array = self.calc_bond_singular_values()
+        out = []
+        for sigma in s_array:
+            p = np.asarray(sigma, dtype=float) ** 2
+            p = p / p.sum()
+            p = p[p > 0]
+            out.append(float(-(p * np.log(p)).sum()))
+        return np.array(out)
+
     # ---- on-disk format (reference mp.py:1085-1113, mps.py:352-386) -------
     def dump(self, fname):
         """Protocol "0.4" (reference mp.py:1085-1113): ``mt_<i>`` site

@@ -76,3 +76,33 @@ def e_occupations_oracle(state, model):
         if b.is_electron:
             out.append(orc.calc_1site_rdm(state, i)[1, 1].real)
     return np.array(out)
+
+
+def reference_holstein_model():
+    """renormalizer/tests/parameter.py:8-30 (3 molecules, two modes each,
+    4 levels, ground / excited frequencies equal)."""
+    from renormalizer_amd.utils import constant
+    elocalex = Quantity(2.67, "eV")
+    j_matrix = np.array([[0.0, -0.1, -0.2], [-0.1, 0.0, -0.3], [-0.2, -0.3, 0.0]]) / constant.au2ev
+    omega_q = [Quantity(106.51, "cm^{-1}"), Quantity(1555.55, "cm^{-1}")]
+    omega = [[omega_q[0], omega_q[0]], [omega_q[1], omega_q[1]]]
+    dis_q = [Quantity(30.1370, "a.u."), Quantity(8.7729, "a.u.")]
+    displacement = [[Quantity(0), dis_q[0]], [Quantity(0), dis_q[1]]]
+    ph_list = [Phonon(*args) for args in zip(omega, displacement, [4, 4])]
+    return HolsteinModel([Mol(elocalex, ph_list, 15.45)] * 3, j_matrix)
+
+
+def mpdm_state(mpdm) -> orc.State:
+    """Oracle state with the reference's 4-leg density-operator sites."""
+    sig = [orc.add_outer(np.asarray(b.sigmaqn), np.zeros_like(np.asarray(b.sigmaqn)))
+           for b in mpdm.model.basis]
+    return orc.State(mpdm.mpdm_sites(), [np.array(q) for q in mpdm.qn], mpdm.qnidx, mpdm.qntot,
+                     mpdm.to_right, sig)
+
+
+def e_occupations_mpdm_oracle(state, model):
+    """<a_i^+ a_i> of a 4-leg purified state, normalised by its norm."""
+    from renormalizer_amd import Mpo, Op
+    nrm2 = orc.mp_norm(state) ** 2
+    return np.array([orc.expectation(state, mpo_arrays(Mpo(model, Op(r"a^\dagger a", dof)))) / nrm2
+                     for dof in model.e_dofs])

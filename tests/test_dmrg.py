@@ -22,17 +22,7 @@ from renormalizer_amd.utils import constant
 from tests import helpers
 
 
-def reference_holstein_model():
-    """renormalizer/tests/parameter.py:8-30 (3 molecules, two modes each,
-    4 levels, ground / excited frequencies equal)."""
-    elocalex = Quantity(2.67, "eV")
-    j_matrix = np.array([[0.0, -0.1, -0.2], [-0.1, 0.0, -0.3], [-0.2, -0.3, 0.0]]) / constant.au2ev
-    omega_q = [Quantity(106.51, "cm^{-1}"), Quantity(1555.55, "cm^{-1}")]
-    omega = [[omega_q[0], omega_q[0]], [omega_q[1], omega_q[1]]]
-    dis_q = [Quantity(30.1370, "a.u."), Quantity(8.7729, "a.u.")]
-    displacement = [[Quantity(0), dis_q[0]], [Quantity(0), dis_q[1]]]
-    ph_list = [Phonon(*args) for args in zip(omega, displacement, [4, 4])]
-    return HolsteinModel([Mol(elocalex, ph_list, 15.45)] * 3, j_matrix)
+reference_holstein_model = helpers.reference_holstein_model
 
 
 PROCEDURE = [[10, 0.4], [20, 0.2], [30, 0.1], [40, 0], [40, 0]]
