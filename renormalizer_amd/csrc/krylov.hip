@@ -20,11 +20,10 @@
 //     per chunk (normally once per call);
 //   * reno_expm_krylov_async queues up to max_matvec iterations plus the
 //     finalisation and returns at once; the outcome lands in a device info
-//     array that the caller reads whenever it likes.  Its convergence checks
-//     (the small exponential and the combination) run on a second stream:
-//     the next matvec needs only the recurrence, so it overlaps them.  The
-//     state block then records the iteration of each kind of exit, and the
-//     finalisation takes the one the reference's loop would reach first.
+//     array that the caller reads whenever it likes.
+// A convergence check first tests one block of 256 elements in every 4096:
+// a violation there settles that allclose fails, which is the usual outcome
+// until the last check, and the full pass over the vectors is skipped.
 #include <math.h>
 #include <string.h>
 
@@ -49,17 +48,18 @@ enum {
   KS_COUNT = 5,    // blocks finished in the current launch (last-block folds)
   KS_BITER = 6,    // reference iteration j of a breakdown / full-space exit (else INT_MAX)
   KS_CONV = 7,     // 1 + iteration j of the converged allclose check (else 0)
-  KS_INTS = 8
+  KS_PFAIL = 8,    // iteration j whose probe found an allclose violation
+  KS_INTS = 12     // the block holds at most 12 ints (scal[2..8))
 };
 
 // How the call ended, from the state block: 1 converged (result in
 // res[RESIDX], CONV matvecs), 2 breakdown / 3 full space (combination of
 // MFINAL vectors, NMATVEC matvecs), 0 iteration budget exhausted, -1 zero v0.
-// The checks may run on a second stream, behind the recurrence, so a check
-// can converge at an iteration after the one where the recurrence already
-// stopped; the earlier event wins, as in the reference's loop (a breakdown
-// and a converged check at the same iteration give the same vector, and the
-// breakdown is taken).
+// A check of iteration j is queued behind the recurrence of iteration j,
+// which can already have found the breakdown of iteration j - 1 (it folds
+// beta_{j-1}); the earlier event wins, as in the reference's loop (a
+// breakdown and a converged check at the same iteration give the same
+// vector, and the breakdown is taken).
 __host__ __device__ __forceinline__ int final_kind(const int* st) {
   const int status = st[KS_STATUS];
   if (status < 0) return -1;
@@ -254,6 +254,20 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// Neighbour exchange across the whole wavefront by DPP wave shifts (one VALU
+// move per 32-bit half, no LDS): lane l receives lane l - 1 (shr) or
+// lane l + 1 (shl); the lane without a source receives 0.
+__device__ __forceinline__ double wave_shr1(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_shl1(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
 constexpr int kChebMax = 512;  // Bessel terms of the Chebyshev branch
 
 // J_k(x), k = 0 .. nmax - 1, for x > 0 by Miller's backward recurrence
@@ -298,7 +312,7 @@ __device__ int bessel_j_miller(double x, int nmax, double* J) {
 //   h = dt / q: every factor has ||h (T - c I)|| <= 4, and K = 34 terms leave
 //   a truncation error below 4^35 / 35! ~ 1e-19.
 // One wavefront; lane l holds entries l*R .. l*R+R-1; neighbours cross lanes
-// by shuffles.  Fixed operation order.
+// by DPP wave shifts.  Fixed operation order.
 // final == 0: the check of iteration m-1 (skipped once stopped);
 // final == 1: end of the call, m = MFINAL, run only for status 2 / 3.
 template <int R>
@@ -362,8 +376,8 @@ __global__ __launch_bounds__(64) void k_tri_coef(const double* __restrict__ alph
       acci[r] = 0.0;
     }
     for (int k = 1; k < K; ++k) {
-      const double lr = __shfl_up(yp[R - 1], 1, 64);
-      const double rr = __shfl_down(yp[0], 1, 64);
+      const double lr = wave_shr1(yp[R - 1]);
+      const double rr = wave_shl1(yp[0]);
       double pv = lr;
       double t[R];
 #pragma unroll
@@ -414,8 +428,8 @@ __global__ __launch_bounds__(64) void k_tri_coef(const double* __restrict__ alph
     for (int k = 1; k <= 34; ++k) {
       // neighbours across lanes: the entry before this lane's first and the
       // one after its last (zero-weighted at the ends via bl / br)
-      const double lr = __shfl_up(xr[R - 1], 1, 64), li = __shfl_up(xi[R - 1], 1, 64);
-      const double rr = __shfl_down(xr[0], 1, 64), ri = __shfl_down(xi[0], 1, 64);
+      const double lr = wave_shr1(xr[R - 1]), li = wave_shr1(xi[R - 1]);
+      const double rr = wave_shl1(xr[0]), ri = wave_shl1(xi[0]);
       const double fr = hr / (double)k, fi = hi_ / (double)k;
       double pr = lr, pi = li;  // old x[r-1]
 #pragma unroll
@@ -442,68 +456,89 @@ __global__ __launch_bounds__(64) void k_tri_coef(const double* __restrict__ alph
   }
 }
 
-// res_new[i] = sum_j coef[j] sig_j V[j][i] and the count of elements
-// violating |res_old - res_new| <= atol + rtol |res_new| (numpy.allclose(
-// res_old, res_new), krylov.py:76-81); res_old may be null (first check: no
-// test).  The last block to finish sees whether any block counted a
-// violation: none means converged at iteration j with the result in res_new
-// (buffer residx).
-__global__ __launch_bounds__(kRedThreads) void k_combine(const double2* __restrict__ V,
-                                                         int64_t n, int64_t vstride, int nv,
-                                                         const double2* __restrict__ coef,
-                                                         const double* __restrict__ sig,
-                                                         const double2* __restrict__ res_old,
-                                                         double2* __restrict__ res_new, int j,
-                                                         int residx, int* st) {
+// Convergence check of iteration j (krylov.py:76-81): with
+//   res_new[i] = sum_{k<nv} coef_new[k] sig_k V[k][i]   (this check) and
+//   res_old[i] = sum_{k<nv-2} coef_old[k] sig_k V[k][i] (the previous one),
+// numpy.allclose(res_old, res_new) holds when no element violates
+// |res_old - res_new| <= atol + rtol |res_new|.  Both combinations are formed
+// from V in one pass (res_old is not kept between checks), in the same
+// order as k_final_out forms a result, so the values are those the stored
+// vectors would hold.
+//   probe == 1: blocks of 256 consecutive elements, one per 4096; any
+//     violation there proves the check fails (st[KS_PFAIL] = j), and the
+//     full pass is skipped;
+//   probe == 0: every element, res_new kept in res (buffer residx); the last
+//     block to finish sees whether any block counted a violation (low 12
+//     bits of the completion atomic: blocks done, the rest: blocks with a
+//     violation): none means converged at iteration j.
+__global__ __launch_bounds__(kRedThreads) void k_check(const double2* __restrict__ V, int64_t n,
+                                                       int64_t vstride, int nv,
+                                                       const double2* __restrict__ coef_new,
+                                                       const double2* __restrict__ coef_old,
+                                                       const double* __restrict__ sig,
+                                                       double2* __restrict__ res, int j, int residx,
+                                                       int probe, int* st) {
   if (check_void(st, j)) return;
+  if (!probe && st[KS_PFAIL] == j) return;
   __shared__ double sh[kRedThreads];
-  __shared__ double2 sc[64];
+  __shared__ double2 sc[2][64];
+  const int nvo = nv - 2;
   for (int k = threadIdx.x; k < nv && k < 64; k += kRedThreads) {
-    const double2 c = coef[k];
-    sc[k] = make_double2(c.x * sig[k], c.y * sig[k]);
+    const double2 c = coef_new[k];
+    sc[0][k] = make_double2(c.x * sig[k], c.y * sig[k]);
+    const double2 o = k < nvo ? coef_old[k] : make_double2(0.0, 0.0);
+    sc[1][k] = make_double2(o.x * sig[k], o.y * sig[k]);
   }
   __syncthreads();
   double viol = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kRedThreads) {
-    double xr = 0.0, xi = 0.0;
+  const int64_t first = probe ? (int64_t)blockIdx.x * 4096 + threadIdx.x
+                              : blockIdx.x * (int64_t)kRedThreads + threadIdx.x;
+  const int64_t step = probe ? n : (int64_t)gridDim.x * kRedThreads;
+  for (int64_t i = first; i < n; i += step) {
+    double xr = 0.0, xi = 0.0, orr = 0.0, oi = 0.0;
     for (int k = 0; k < nv; ++k) {
-      double2 c;
+      double2 c, o;
       if (k < 64) {
-        c = sc[k];
+        c = sc[0][k];
+        o = sc[1][k];
       } else {
-        c = coef[k];
+        c = coef_new[k];
         c.x *= sig[k];
         c.y *= sig[k];
+        o = k < nvo ? coef_old[k] : make_double2(0.0, 0.0);
+        o.x *= sig[k];
+        o.y *= sig[k];
       }
       const double2 v = V[(int64_t)k * vstride + i];
       xr += v.x * c.x - v.y * c.y;
       xi += v.x * c.y + v.y * c.x;
-    }
-    res_new[i] = make_double2(xr, xi);
-    if (res_old) {
-      const double2 o = res_old[i];
-      const double dr = o.x - xr, di = o.y - xi;
-      const double lhs = sqrt(dr * dr + di * di);
-      const double rhs = 1e-8 + 1e-5 * sqrt(xr * xr + xi * xi);
-      if (!(lhs <= rhs)) viol += 1.0;
-    }
-  }
-  if (!res_old) return;
-  // the verdict travels in the same atomic as the completion count (low 12
-  // bits: blocks done, the rest: blocks with a violation), so no partial sums
-  // and no device-wide fence are needed
-  const double sv = block_sum(viol, sh);
-  if (threadIdx.x == 0) {
-    const int inc = 1 + (sv > 0.0 ? (1 << 12) : 0);
-    const int tot = atomicAdd(st + KS_COUNT, inc) + inc;
-    if ((tot & ((1 << 12) - 1)) == (int)gridDim.x) {
-      st[KS_COUNT] = 0;
-      if ((tot >> 12) == 0) {
-        st[KS_RESIDX] = residx;
-        st[KS_CONV] = j + 1;
-        st[KS_STOP] = 1;
+      if (k < nvo) {
+        orr += v.x * o.x - v.y * o.y;
+        oi += v.x * o.y + v.y * o.x;
       }
+    }
+    if (!probe) res[i] = make_double2(xr, xi);
+    const double dr = orr - xr, di = oi - xi;
+    const double lhs = sqrt(dr * dr + di * di);
+    const double rhs = 1e-8 + 1e-5 * sqrt(xr * xr + xi * xi);
+    if (!(lhs <= rhs)) viol += 1.0;
+  }
+  const double sv = block_sum(viol, sh);
+  if (threadIdx.x != 0) return;
+  if (probe) {
+    if (sv > 0.0) st[KS_PFAIL] = j;
+    return;
+  }
+  // the verdict travels in the same atomic as the completion count, so no
+  // partial sums and no device-wide fence are needed
+  const int inc = 1 + (sv > 0.0 ? (1 << 12) : 0);
+  const int tot = atomicAdd(st + KS_COUNT, inc) + inc;
+  if ((tot & ((1 << 12) - 1)) == (int)gridDim.x) {
+    st[KS_COUNT] = 0;
+    if ((tot >> 12) == 0) {
+      st[KS_RESIDX] = residx;
+      st[KS_CONV] = j + 1;
+      st[KS_STOP] = 1;
     }
   }
 }
@@ -574,7 +609,7 @@ static KrylovLayout krylov_layout(const reno_hop_desc* op, int kcap) {
   k.off_part = o; o += align_up(kMaxPartials * sizeof(double));
   k.off_part2 = o; o += align_up(2 * kMaxPartials * sizeof(double));
   k.off_scal = o; o += align_up((size_t)(3 * kcap + 17) * sizeof(double));
-  k.off_coef = o; o += align_up((size_t)kcap * 16);
+  k.off_coef = o; o += align_up((size_t)3 * kcap * 16);  // two checks, the final
   k.total = o;
   return k;
 }
@@ -591,41 +626,10 @@ struct HostStage {
 };
 static thread_local HostStage t_stage;
 
-// The convergence checks of the asynchronous entry point run on a second
-// stream of the same device (default priority, non-blocking), so the next
-// matvec does not wait for them; one event hands each check its recurrence,
-// another hands the finalisation the last check.  Created on first use, per
-// thread and device, and kept until the process ends.
-struct CheckStream {
-  hipStream_t s = nullptr;
-  hipEvent_t fwd = nullptr, back = nullptr;
-};
-constexpr int kMaxDevices = 64;
-static thread_local CheckStream t_check[kMaxDevices];
-
-static int check_stream(CheckStream** out) {
-  int dev = 0;
-  RENO_HIP_CHECK(hipGetDevice(&dev));
-  if (dev < 0 || dev >= kMaxDevices) {
-    set_error("expm_krylov_async: device %d beyond the supported %d", dev, kMaxDevices);
-    return RENO_EINVAL;
-  }
-  CheckStream& c = t_check[dev];
-  if (!c.s) {
-    RENO_HIP_CHECK(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
-    RENO_HIP_CHECK(hipEventCreateWithFlags(&c.fwd, hipEventDisableTiming));
-    RENO_HIP_CHECK(hipEventCreateWithFlags(&c.back, hipEventDisableTiming));
-  }
-  *out = &c;
-  return RENO_OK;
-}
-
 // One call's device buffers and launch parameters.
 struct KrylovRun {
   const reno_hop_desc* op;
   hipStream_t s;
-  hipStream_t cs;              // stream of the convergence checks (s, or a CheckStream)
-  hipEvent_t ev_fwd, ev_back;  // only with a separate check stream
   int64_t n;
   int kcap;
   KrylovLayout L;
@@ -656,8 +660,6 @@ static void bind_run(KrylovRun& r, const reno_hop_desc* op, hipStream_t s, void*
                      double dt_re, double dt_im) {
   r.op = op;
   r.s = s;
-  r.cs = s;
-  r.ev_fwd = r.ev_back = nullptr;
   r.n = hop_vec_len(op);
   r.kcap = kcap;
   r.L = krylov_layout(op, kcap);
@@ -700,14 +702,16 @@ static int queue_start(KrylovRun& r, const void* v0) {
   return RENO_OK;
 }
 
-// k_tri_coef instantiation for up to m entries (checks on the check stream,
-// the final one on the main stream)
-static void launch_tri_coef(KrylovRun& r, int m, int final) {
-  hipStream_t st = final ? r.s : r.cs;
+// coefficient buffer: 0 / 1 for the checks (alternating, so a check sees
+// the previous one's coefficients), 2 for the final combination
+static double2* coef_buf(KrylovRun& r, int which) { return r.d_coef + (size_t)which * r.kcap; }
+
+// k_tri_coef instantiation for up to m entries
+static void launch_tri_coef(KrylovRun& r, int m, int final, double2* coef) {
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(1), dim3(64), 0, st, (const double*)r.d_alpha,
+    hipLaunchKernelGGL(kern, dim3(1), dim3(64), 0, r.s, (const double*)r.d_alpha,
                        (const double*)r.d_beta, (const double*)r.scal, r.dt_re, r.dt_im, m, final,
-                       (const int*)r.st, r.d_coef);
+                       (const int*)r.st, coef);
   };
   if (m <= 64) go(k_tri_coef<1>);
   else if (m <= 128) go(k_tri_coef<2>);
@@ -746,17 +750,19 @@ static int queue_iter(KrylovRun& r, int64_t j) {
                      r.d_sig, (int)j, r.n, r.thr, nnext, r.st);
   if (j > 3 && j % 2 == 0) {
     const int m = (int)j + 1;
-    const int dst = (int)((j / 2) & 1);
-    const bool compare = j > 4;
-    if (r.cs != r.s) {
-      RENO_HIP_CHECK(hipEventRecord(r.ev_fwd, r.s));
-      RENO_HIP_CHECK(hipStreamWaitEvent(r.cs, r.ev_fwd, 0));
+    const int cb = (int)((j / 2) & 1);
+    launch_tri_coef(r, m, 0, coef_buf(r, cb));
+    if (j > 4) {  // the first check has nothing to compare with
+      const int nprobe = (int)((r.n + 4095) / 4096);
+      hipLaunchKernelGGL(k_check, dim3(nprobe), dim3(kRedThreads), 0, r.s, (const double2*)r.V, r.n,
+                         (int64_t)r.vstride, m, (const double2*)coef_buf(r, cb),
+                         (const double2*)coef_buf(r, cb ^ 1), (const double*)r.d_sig, r.res[cb],
+                         (int)j, cb, 1, r.st);
+      hipLaunchKernelGGL(k_check, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.V, r.n,
+                         (int64_t)r.vstride, m, (const double2*)coef_buf(r, cb),
+                         (const double2*)coef_buf(r, cb ^ 1), (const double*)r.d_sig, r.res[cb],
+                         (int)j, cb, 0, r.st);
     }
-    launch_tri_coef(r, m, 0);
-    hipLaunchKernelGGL(k_combine, dim3(r.np), dim3(kRedThreads), 0, r.cs, (const double2*)r.V, r.n,
-                       (int64_t)r.vstride, m, (const double2*)r.d_coef, (const double*)r.d_sig,
-                       compare ? (const double2*)r.res[dst ^ 1] : (const double2*)nullptr,
-                       r.res[dst], (int)j, dst, r.st);
   }
   RENO_CHECK_LAUNCH();
   return RENO_OK;
@@ -764,13 +770,9 @@ static int queue_iter(KrylovRun& r, int64_t j) {
 
 // final combination / copy into out; m_max bounds MFINAL
 static int queue_final(KrylovRun& r, void* out, int m_max) {
-  if (r.cs != r.s) {
-    RENO_HIP_CHECK(hipEventRecord(r.ev_back, r.cs));
-    RENO_HIP_CHECK(hipStreamWaitEvent(r.s, r.ev_back, 0));
-  }
-  launch_tri_coef(r, m_max, 1);
+  launch_tri_coef(r, m_max, 1, coef_buf(r, 2));
   hipLaunchKernelGGL(k_final_out, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)r.V,
-                     r.n, (int64_t)r.vstride, (const double2*)r.d_coef, (const double*)r.d_sig,
+                     r.n, (int64_t)r.vstride, (const double2*)coef_buf(r, 2), (const double*)r.d_sig,
                      (const double2*)r.res[0], (const double2*)r.res[1], (double2*)out,
                      (const int*)r.st);
   RENO_CHECK_LAUNCH();
@@ -869,11 +871,6 @@ extern "C" int reno_expm_krylov_async(const reno_hop_desc* op, double dt_re, dou
   }
   KrylovRun r;
   bind_run(r, op, s, ws, kcap, dt_re, dt_im);
-  CheckStream* cs = nullptr;
-  RENO_TRY(check_stream(&cs));
-  r.cs = cs->s;
-  r.ev_fwd = cs->fwd;
-  r.ev_back = cs->back;
   RENO_TRY(queue_start(r, v0));
   for (int64_t j = 0; j < jmax; ++j) RENO_TRY(queue_iter(r, j));
   RENO_TRY(queue_final(r, out, (int)jmax));
