@@ -332,3 +332,25 @@ def test_tdvp_ps2_product_state_exact():
         for s in dev.numpy_sites():
             psi = np.tensordot(psi, s, axes=1).reshape(-1, s.shape[-1])
         np.testing.assert_allclose(psi[:, 0], exact[k], atol=1e-7)
+
+
+def test_holstein_disorder_c3_shape_parity():
+    """The C3 workload shape, three of its instances: Holstein chains of 8,
+    14 and 20 molecules (scheme 2, phonon d = 8), site energies drawn from
+    N(0, 0.5) with a fixed seed, J = -1, omega = 1, g = 0.7, Franck-Condon
+    electron in the middle, bond dimensions 16, 32 and 48, one-electron
+    quantum numbers, dt = 0.02.  Two steps each, FP64 device vs oracle:
+    populations to 1e-8, matvec counts and quantum numbers."""
+    rng = np.random.default_rng(2024)
+    for nmol, m in ((8, 16), (14, 32), (20, 48)):
+        print(f"[tdvp C3 instance {nmol} molecules M={m}]", flush=True)
+        elocal = rng.normal(0.0, 0.5, nmol)
+        model = helpers.holstein_model(nmol, 8, j=-1.0, g=0.7, elocal=list(elocal))
+        mps, mpo = helpers.holstein_initial(model, m)
+        obs_d = lambda x: x.e_occupations
+        obs_o = lambda s, model=model: helpers.e_occupations_oracle(s, model)
+        od, oo, sd, so, dmps, st = _run_pair(mps, mpo, 0.02, 2, obs_d, obs_o)
+        np.testing.assert_allclose(od, oo, atol=1e-8, rtol=0)
+        np.testing.assert_allclose(od.sum(axis=1), 1.0, atol=1e-10)
+        _assert_steps_match(sd, so)
+        assert [np.array(q).tolist() for q in dmps.qn] == [np.array(q).tolist() for q in st.qn]
