@@ -20,13 +20,14 @@
 
 namespace reno {
 
-static std::mutex g_handle_mu;
-static std::map<int, rocblas_handle> g_handles;
+// One rocBLAS handle per host thread and device: a handle carries the stream
+// it was last set to, so threads driving different streams must not share
+// one.  Kept until the process ends.
+static thread_local std::map<int, rocblas_handle> g_handles;
 
 static int get_handle(rocblas_handle* out) {
   int dev = 0;
   RENO_HIP_CHECK(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(g_handle_mu);
   auto it = g_handles.find(dev);
   if (it != g_handles.end()) {
     *out = it->second;
