@@ -21,7 +21,7 @@ from renormalizer_amd.mps.mps import Mps
 
 
 def evolve_batch(mps_list: Sequence[Mps], mpo: Union[Mpo, Sequence[Mpo]], evolve_dt,
-                 nstreams: int = 4) -> List[Mps]:
+                 nstreams: int = 1) -> List[Mps]:
     """``[m.evolve(mpo_i, evolve_dt) for m in mps_list]`` with the work of
     state i on stream i mod nstreams.  ``mpo`` is one operator for all states
     or one per state.  The caller's stream waits for every stream before the
