@@ -195,8 +195,9 @@ int reno_prof_collect(int nkinds, double* ms, double* flop_timed, int* timed, do
  * own (queue latency), not the kernel. */
 int reno_prof_cancelled(int nkinds, double* ms, int* count);
 /* Number of GEMM kernel instantiations; for 0 <= kind < that number, fills
- * info[0..5] = precision, A k-contiguous, B k-contiguous, and the storage
- * kinds of A, B, C (0 complex128, 1 complex64, 2 float64, 3 float32). */
+ * info[0..6] = precision, A k-contiguous, B k-contiguous, the storage kinds
+ * of A, B, C (0 complex128, 1 complex64, 2 float64, 3 float32) and the
+ * square output tile (64, or 32 for products with few 64 x 64 tiles). */
 int reno_prof_kind_info(int kind, int* info);
 
 /* ---- test hook: one v_mfma 16x16x4 tile, used by the lane-layout check.

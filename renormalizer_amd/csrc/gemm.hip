@@ -100,18 +100,25 @@ __device__ __forceinline__ void store_elem(void* base, int kind, int64_t off, R 
   else store_k<R, ST_C64>(base, off, re, im, beta);
 }
 
-constexpr int kBM = 64;
+constexpr int kBM = 64;  // tile of k_cgemm; k_cgemm_s uses 32 x 32
 constexpr int kBN = 64;
 constexpr int kBK = 16;
 constexpr int kThreads = 256;
 constexpr int kPad = 1;  // odd row stride: conflict-free LDS writes along k
-constexpr int kPerThread = (kBM * kBK) / kThreads;  // 4 (A) and 4 (B)
+constexpr int kSmallTile = 32;
+constexpr int64_t kSmallGridTiles = 64;  // 64 x 64 tiles (over the batch) up to which 32 x 32 is used
 
 // AK: A is contiguous along k (sAk == 1), else along m.
 // BKC: B is contiguous along k (sBk == 1), else along n.
 // AREAL: A has no imaginary part.
-template <typename R, bool AK, bool BKC, int AKIND, int BKIND, int CKIND>
-__global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
+// TM: square output tile (64: four waves of 32 x 32 = 2 x 2 MFMA blocks;
+// 32: four waves of 16 x 16 = one block, for products whose 64 x 64 tile
+// grid would leave most of the chip idle).
+template <typename R, bool AK, bool BKC, int AKIND, int BKIND, int CKIND, int TM>
+__device__ __forceinline__ void cgemm_body(const GemmCall& g) {
+  constexpr int kBM = TM, kBN = TM;
+  constexpr int MB = TM / 32;  // MFMA blocks per wave along m and n
+  constexpr int kPerThread = (kBM * kBK) / kThreads;  // A and B elements per thread
   constexpr bool AREAL = (AKIND == ST_F64 || AKIND == ST_F32);
   // double-buffered: [buffer][re / im plane][k][m or n]
   __shared__ R sA[2][2][kBK][kBM + kPad];
@@ -150,11 +157,11 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
   // (Gauss): P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); then
   // Re = P1 - P2, Im = P3 - P1 - P2.  cr holds P1, ci holds P3 and cq P2.
   constexpr bool K3M = std::is_same<R, double>::value && !AREAL;
-  acc_t cr[2][2], ci[2][2], cq[2][2];
+  acc_t cr[MB][MB], ci[MB][MB], cq[MB][MB];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < MB; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < MB; ++j) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         cr[i][j][r] = (R)0;
@@ -263,36 +270,36 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
 #pragma unroll
       for (int ks = 0; ks < kBK / 4; ++ks) {
         const int kr = ks * 4 + kq;
-        R a_re[2], a_im[2], b_re[2], b_im[2];
+        R a_re[MB], a_im[MB], b_re[MB], b_im[MB];
 #pragma unroll
-        for (int mb = 0; mb < 2; ++mb) {
-          a_re[mb] = sA[cur][0][kr][wm * 32 + mb * 16 + r16];
-          a_im[mb] = AREAL ? (R)0 : sA[cur][1][kr][wm * 32 + mb * 16 + r16];
+        for (int mb = 0; mb < MB; ++mb) {
+          a_re[mb] = sA[cur][0][kr][wm * (TM / 2) + mb * 16 + r16];
+          a_im[mb] = AREAL ? (R)0 : sA[cur][1][kr][wm * (TM / 2) + mb * 16 + r16];
         }
 #pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-          b_re[nb] = sB[cur][0][kr][wn * 32 + nb * 16 + r16];
-          b_im[nb] = sB[cur][1][kr][wn * 32 + nb * 16 + r16];
+        for (int nb = 0; nb < MB; ++nb) {
+          b_re[nb] = sB[cur][0][kr][wn * (TM / 2) + nb * 16 + r16];
+          b_im[nb] = sB[cur][1][kr][wn * (TM / 2) + nb * 16 + r16];
         }
         if constexpr (K3M) {
-          R a_s[2], b_s[2];
+          R a_s[MB], b_s[MB];
 #pragma unroll
-          for (int mb = 0; mb < 2; ++mb) a_s[mb] = a_re[mb] + a_im[mb];
+          for (int mb = 0; mb < MB; ++mb) a_s[mb] = a_re[mb] + a_im[mb];
 #pragma unroll
-          for (int nb = 0; nb < 2; ++nb) b_s[nb] = b_re[nb] + b_im[nb];
+          for (int nb = 0; nb < MB; ++nb) b_s[nb] = b_re[nb] + b_im[nb];
 #pragma unroll
-          for (int mb = 0; mb < 2; ++mb)
+          for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-            for (int nb = 0; nb < 2; ++nb) {
+            for (int nb = 0; nb < MB; ++nb) {
               cr[mb][nb] = Mfma<R>::mac(a_re[mb], b_re[nb], cr[mb][nb]);
               cq[mb][nb] = Mfma<R>::mac(a_im[mb], b_im[nb], cq[mb][nb]);
               ci[mb][nb] = Mfma<R>::mac(a_s[mb], b_s[nb], ci[mb][nb]);
             }
         } else {
 #pragma unroll
-          for (int mb = 0; mb < 2; ++mb)
+          for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-            for (int nb = 0; nb < 2; ++nb) {
+            for (int nb = 0; nb < MB; ++nb) {
               cr[mb][nb] = Mfma<R>::mac(a_re[mb], b_re[nb], cr[mb][nb]);
               ci[mb][nb] = Mfma<R>::mac(a_re[mb], b_im[nb], ci[mb][nb]);
               if (!AREAL) {
@@ -313,13 +320,13 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
     run(std::integral_constant<bool, true>());
 
 #pragma unroll
-  for (int mb = 0; mb < 2; ++mb)
+  for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb)
+    for (int nb = 0; nb < MB; ++nb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * 32 + mb * 16 + Mfma<R>::out_row(kq, r);
-        const int col = n0 + wn * 32 + nb * 16 + r16;
+        const int row = m0 + wm * (TM / 2) + mb * 16 + Mfma<R>::out_row(kq, r);
+        const int col = n0 + wn * (TM / 2) + nb * 16 + r16;
         R re = cr[mb][nb][r], im = ci[mb][nb][r];
         if constexpr (K3M) {
           re = cr[mb][nb][r] - cq[mb][nb][r];
@@ -330,6 +337,15 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
       }
 }
 
+template <typename R, bool AK, bool BKC, int AKIND, int BKIND, int CKIND>
+__global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
+  cgemm_body<R, AK, BKC, AKIND, BKIND, CKIND, kBM>(g);
+}
+template <typename R, bool AK, bool BKC, int AKIND, int BKIND, int CKIND>
+__global__ __launch_bounds__(kThreads) void k_cgemm_s(GemmCall g) {
+  cgemm_body<R, AK, BKC, AKIND, BKIND, CKIND, kSmallTile>(g);
+}
+
 // ---- instantiations -----------------------------------------------------
 // The (precision, A k-contiguous, B k-contiguous, A/B/C storage) combinations
 // the contractions use; launch_gemm rejects any other.  Their order is the
@@ -337,10 +353,11 @@ __global__ __launch_bounds__(kThreads) void k_cgemm(GemmCall g) {
 typedef void (*GemmKernel)(GemmCall);
 struct GemmKind {
   int prec, ak, bk, akind, bkind, ckind;
-  GemmKernel fn;
+  GemmKernel fn;    // 64 x 64 tiles
+  GemmKernel fn_s;  // 32 x 32 tiles
 };
 #define RENO_KIND(R, P, AK, BK, A, B, C) \
-  {P, AK, BK, A, B, C, k_cgemm<R, (bool)AK, (bool)BK, A, B, C>}
+  {P, AK, BK, A, B, C, k_cgemm<R, (bool)AK, (bool)BK, A, B, C>, k_cgemm_s<R, (bool)AK, (bool)BK, A, B, C>}
 static const GemmKind kGemmKinds[] = {
     // FP64: complex128 operands, real MPO stage
     RENO_KIND(double, RENO_PREC_FP64, 1, 0, ST_Z128, ST_Z128, ST_Z128),
@@ -383,8 +400,8 @@ static int find_kind(const GemmCall& g, bool ak, bool bk) {
 // CU: 512 on the 256 CUs) times the K extent of one workgroup plus a fixed
 // per-workgroup cost (prologue, tile store) of about 48 K steps; chunks stay
 // at least 64 deep.
-static void split_plan(int M, int N, int K, int batch, int* nsplit, int* kchunk) {
-  const int64_t tiles = (int64_t)((M + kBM - 1) / kBM) * ((N + kBN - 1) / kBN) * batch;
+static void split_plan(int M, int N, int K, int batch, int* nsplit, int* kchunk, int tile = kBM) {
+  const int64_t tiles = (int64_t)((M + tile - 1) / tile) * ((N + tile - 1) / tile) * batch;
   *nsplit = 1;
   *kchunk = K > 0 ? K : 1;
   constexpr int64_t kSlots = 512;
@@ -571,7 +588,11 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
     set_error("gemm: B must be complex");
     return RENO_EINVAL;
   }
-  const int64_t tiles = (int64_t)((g.M + kBM - 1) / kBM) * ((g.N + kBN - 1) / kBN);
+  // 32 x 32 tiles when the 64 x 64 grid is small (the bond-matrix products
+  // of the zero-site step): four times the workgroups, less split-K
+  const int64_t tiles64 = (int64_t)((g.M + kBM - 1) / kBM) * ((g.N + kBN - 1) / kBN) * g.batch;
+  const int tile = tiles64 <= kSmallGridTiles ? kSmallTile : kBM;
+  const int64_t tiles = (int64_t)((g.M + tile - 1) / tile) * ((g.N + tile - 1) / tile);
   if (tiles > 0x7fffffff) {
     set_error("gemm: too many tiles");
     return RENO_EINVAL;
@@ -585,7 +606,7 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
   const bool bk = (g.sBk == 1);
   // split K when the tile grid cannot fill the chip and scratch is given
   int nsplit = 1, kchunk = g.K > 0 ? g.K : 1;
-  split_plan(g.M, g.N, g.K, g.batch, &nsplit, &kchunk);
+  split_plan(g.M, g.N, g.K, g.batch, &nsplit, &kchunk, tile);
   const size_t need = nsplit > 1 ? (size_t)nsplit * g.batch * g.M * g.N * elem_bytes(g.prec) : 0;
   if (nsplit > 1 && (!g.part || g.part_bytes < need)) {
     nsplit = 1;
@@ -616,6 +637,7 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
     return RENO_EINVAL;
   }
   dim3 grid((unsigned)tiles, (unsigned)g.batch, (unsigned)nsplit);
+  const GemmKernel kfn = tile == kSmallTile ? kGemmKinds[kid].fn_s : kGemmKinds[kid].fn;
   // K == 0 needs no special case: the kernel stores its zero accumulators.
   ProfRec rec{nullptr, nullptr, 0.0, 0, -1};
   bool prof = false, timed = false;
@@ -642,10 +664,10 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
       // the events are bound to this dispatch: they time the matrix-core
       // kernel alone, without the queue latency in front of it (the split-K
       // reduction is a separate, bandwidth-bound kernel)
-      RENO_HIP_CHECK(hipExtLaunchKernel((const void*)kGemmKinds[kid].fn, grid, dim3(kThreads), args,
+      RENO_HIP_CHECK(hipExtLaunchKernel((const void*)kfn, grid, dim3(kThreads), args,
                                         0, stream, rec.a, rec.b, 0));
     } else {
-      RENO_HIP_CHECK(hipLaunchKernel((const void*)kGemmKinds[kid].fn, grid, dim3(kThreads), args, 0,
+      RENO_HIP_CHECK(hipLaunchKernel((const void*)kfn, grid, dim3(kThreads), args, 0,
                                      stream));
     }
   }
@@ -682,7 +704,8 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
   }
   if (prof) {
     rec.flop = (areal ? 4.0 : 8.0) * (double)g.M * (double)g.N * (double)g.K * (double)g.batch;
-    rec.kind = kid;
+    // timing-hook kinds: [0, n) 64 x 64 tiles, [n, 2n) 32 x 32 tiles
+    rec.kind = tile == kSmallTile ? kid + kNumGemmKinds : kid;
     if (!timed) rec.a = rec.b = nullptr;
     std::lock_guard<std::mutex> lk(g_prof_mu);
     g_prof.push_back(rec);
@@ -692,7 +715,8 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
 
 size_t splitk_bytes(int prec, int M, int N, int K, int batch) {
   int nsplit = 1, kchunk = 1;
-  split_plan(M, N, K, batch, &nsplit, &kchunk);
+  const int64_t tiles64 = (int64_t)((M + kBM - 1) / kBM) * ((N + kBN - 1) / kBN) * batch;
+  split_plan(M, N, K, batch, &nsplit, &kchunk, tiles64 <= kSmallGridTiles ? kSmallTile : kBM);
   return nsplit > 1 ? (size_t)nsplit * batch * M * N * elem_bytes(prec) : 0;
 }
 
@@ -809,16 +833,17 @@ extern "C" int reno_probe_mfma16(int prec, const void* a, const void* b, void* c
 extern "C" int reno_prof_enable(int period) { return reno::prof_enable(period); }
 
 extern "C" int reno_prof_kind_info(int kind, int* info) {
-  if (kind >= 0 && kind < reno::kNumGemmKinds && info) {
-    const reno::GemmKind& k = reno::kGemmKinds[kind];
+  if (kind >= 0 && kind < 2 * reno::kNumGemmKinds && info) {
+    const reno::GemmKind& k = reno::kGemmKinds[kind % reno::kNumGemmKinds];
     info[0] = k.prec;
     info[1] = k.ak;
     info[2] = k.bk;
     info[3] = k.akind;
     info[4] = k.bkind;
     info[5] = k.ckind;
+    info[6] = kind < reno::kNumGemmKinds ? reno::kBM : reno::kSmallTile;
   }
-  return reno::kNumGemmKinds;
+  return 2 * reno::kNumGemmKinds;
 }
 
 extern "C" int reno_prof_cancelled(int nkinds, double* ms, int* count) {

@@ -45,10 +45,11 @@ def gemm_kind_names():
     l = lib()
     n = l.reno_prof_kind_info(-1, None)
     names = []
-    info = (ctypes.c_int * 6)()
+    info = (ctypes.c_int * 7)()
     for k in range(n):
         l.reno_prof_kind_info(k, info)
-        names.append("void reno::k_cgemm<%s, %s, %s, %d, %d, %d>(reno::GemmCall)" % (
+        names.append("void reno::%s<%s, %s, %s, %d, %d, %d>(reno::GemmCall)" % (
+            "k_cgemm" if info[6] == 64 else "k_cgemm_s",
             "double" if info[0] == PREC_FP64 else "float", "true" if info[1] else "false",
             "true" if info[2] else "false", info[3], info[4], info[5]))
     return names
