@@ -680,14 +680,22 @@ class Mps(MatrixProduct):
 
     @classmethod
     def load(cls, model: Model, fname: str):
-        """Reads protocols "0.1", "0.3" and "0.4" (reference mps.py:352-386)
-        without unpickling: the bond quantum numbers come from the
-        ``subqn_<i>`` arrays the reference writes beside its pickled ``qn``.
-        Protocol "0.2" keeps its coefficient in a pickled array and is
-        refused, as is an unknown version."""
+        """Reads files of protocol "0.4" (reference mp.py:1085-1113,
+        mps.py:352-386) without unpickling: the bond quantum numbers come from
+        the ``subqn_<i>`` arrays that protocol writes beside its pickled
+        ``qn``.  Older protocols ("0.1", "0.2", "0.3") keep the quantum
+        numbers only in the pickled object array and are refused with a
+        ValueError, as is an unknown version."""
         with np.load(fname, allow_pickle=False) as f:
             version = str(f["version"])
+            if version not in ("0.1", "0.2", "0.3", "0.4"):
+                raise ValueError(f"Unknown dump version: {version}")
             n = int(f["nsites"])
+            missing = [i for i in range(n + 1) if f"subqn_{i}" not in f.files]
+            if missing:
+                raise ValueError(
+                    f"dump protocol {version} file without subqn_<i> arrays: its quantum numbers "
+                    f"are stored only pickled, which is not read")
             mps = cls()
             mps.model = model
             mps._mp = [f[f"mt_{i}"] for i in range(n)]
@@ -701,10 +709,8 @@ class Mps(MatrixProduct):
             elif version in ("0.3", "0.4"):
                 mps.to_right = bool(f["to_right"])
                 mps.coeff = f["coeff"].item(0)
-            elif version == "0.2":
-                raise ValueError("dump protocol 0.2 stores its coefficient pickled; not read")
             else:
-                raise ValueError(f"Unknown dump version: {version}")
+                raise ValueError("dump protocol 0.2 stores its coefficient pickled; not read")
         return mps
 
     def __str__(self):

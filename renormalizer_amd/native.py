@@ -6,6 +6,7 @@ stream handle are handed to the library, which launches asynchronously on
 that stream.  There is no fallback: if the library is missing, or no GPU is
 present, every compute entry point raises.
 """
+import collections
 import ctypes
 import os
 import threading
@@ -170,21 +171,36 @@ def _stream():
 
 
 class _Workspace:
-    """Grow-only device scratch shared by consecutive calls on one stream."""
+    """Grow-only device scratch, one buffer per (device, stream).
+
+    The Krylov state block, the basis V and the split-K partials live in
+    this buffer, so two streams must never share one: each stream gets its
+    own.  A buffer that grows is released to torch's caching allocator,
+    which hands its memory out again only to allocations on the same
+    stream, i.e. after the kernels already queued there that read it."""
 
     def __init__(self):
         self._buf = {}
+        self._lock = threading.Lock()
 
-    def get(self, nbytes: int, device):
-        key = (device.index if device.index is not None else torch.cuda.current_device())
-        buf = self._buf.get(key)
-        if buf is None or buf.numel() < nbytes:
-            size = max(int(nbytes), 1 << 20)
-            if buf is not None:
-                size = max(size, int(buf.numel() * 1.25))
-            buf = torch.empty(size, dtype=torch.uint8, device=device)
-            self._buf[key] = buf
-        return buf
+    def get(self, nbytes: int, device, stream=None):
+        dev = device.index if device.index is not None else torch.cuda.current_device()
+        if stream is None:
+            stream = torch.cuda.current_stream(dev).cuda_stream
+        key = (dev, int(stream))
+        with self._lock:
+            buf = self._buf.get(key)
+            if buf is None or buf.numel() < nbytes:
+                size = max(int(nbytes), 1 << 20)
+                if buf is not None:
+                    size = max(size, int(buf.numel() * 1.25))
+                buf = torch.empty(size, dtype=torch.uint8, device=device)
+                self._buf[key] = buf
+            return buf
+
+    def keys(self):
+        with self._lock:
+            return list(self._buf)
 
 
 workspace = _Workspace()
@@ -303,21 +319,40 @@ def expm_krylov(L, W, R, v, dt, prec=PREC_FP64):
     desc = _krylov_desc(L, W, R, v, prec)
     dt = complex(dt)
     l = lib()
+    return _expm_krylov_sync(desc, dt, v, l.reno_krylov_workspace_bytes, l.reno_expm_krylov,
+                             lambda nbytes: workspace.get(nbytes, v.device), _stream())
+
+
+def _expm_krylov_sync(desc, dt, v, ws_bytes_fn, call_fn, get_ws, stream):
+    """The retry loop of the synchronous Krylov call: on RENO_ENOSPACE the
+    workspace grows to the size the library asks for (twice the Krylov
+    vectors) until KRYLOV_MAX vectors; a call that has not converged by then
+    raises instead of retrying forever."""
     kcap = _KRYLOV_KCAP
     out = torch.empty_like(v)
     nmv = ctypes.c_int(0)
     needed = ctypes.c_size_t(0)
     while True:
-        nbytes = l.reno_krylov_workspace_bytes(ctypes.byref(desc), kcap)
-        ws = workspace.get(nbytes, v.device)
-        st = l.reno_expm_krylov(ctypes.byref(desc), dt.real, dt.imag, _ptr(v), _ptr(out),
-                                ctypes.byref(nmv), ctypes.byref(needed), _ptr(ws), ws.numel(),
-                                _stream())
+        nbytes = ws_bytes_fn(ctypes.byref(desc), kcap)
+        if nbytes == 0:
+            raise RenoNativeError(f"expm_krylov: no workspace size for {kcap} Krylov vectors")
+        nbytes = max(int(nbytes), int(needed.value))
+        ws = get_ws(nbytes)
+        st = call_fn(ctypes.byref(desc), dt.real, dt.imag, _ptr(v), _ptr(out),
+                     ctypes.byref(nmv), ctypes.byref(needed), _ptr(ws), ws.numel(), stream)
         if st == RENO_ENOSPACE:
-            kcap *= 2
+            if kcap >= KRYLOV_MAX:
+                raise RenoNativeError(
+                    f"expm_krylov: no convergence within {KRYLOV_MAX} Lanczos vectors "
+                    f"(non-finite environment or vector?)")
+            kcap = min(2 * kcap, KRYLOV_MAX)
             continue
         check(st, "reno_expm_krylov")
         return out, nmv.value
+
+
+# largest Krylov space the library supports (kMaxKrylov in krylov.hip)
+KRYLOV_MAX = 1024
 
 
 # largest Krylov space of one asynchronous call
@@ -401,13 +436,7 @@ def block_qr(system, coef, blocks, v_transposed=False):
     if len(blocks) == 0:
         raise ValueError("Invalid quantum number")
     dev = coef.device
-    cblocks = (QnBlock * len(blocks))()
-    kt = 0
-    for i, (r, c) in enumerate(blocks):
-        rt = _device_index(r, dev)
-        ct = _device_index(c, dev)
-        cblocks[i] = QnBlock(len(r), len(c), rt.data_ptr(), ct.data_ptr())
-        kt += min(len(r), len(c))
+    cblocks, keep, kt = qn_block_descs(blocks, dev)
     u = torch.empty((rows, kt), dtype=torch.complex128, device=dev)
     v = torch.empty((kt, cols) if v_transposed else (cols, kt), dtype=torch.complex128, device=dev)
     l = lib()
@@ -419,22 +448,48 @@ def block_qr(system, coef, blocks, v_transposed=False):
     return u, v
 
 
-_INDEX_CACHE = {}
+def qn_block_descs(blocks, dev):
+    """ctypes descriptors of the QN blocks with device index arrays.
+
+    Returns (descriptors, keep, kt): ``keep`` holds every index tensor the
+    descriptors point to, so that none is freed (and its memory handed to
+    another tensor) before the call that reads them has been queued, even
+    if the cache below evicts it meanwhile."""
+    cblocks = (QnBlock * len(blocks))()
+    keep = []
+    kt = 0
+    for i, (r, c) in enumerate(blocks):
+        rt = _device_index(r, dev)
+        ct = _device_index(c, dev)
+        keep += [rt, ct]
+        cblocks[i] = QnBlock(len(r), len(c), rt.data_ptr(), ct.data_ptr())
+        kt += min(len(r), len(c))
+    return cblocks, keep, kt
+
+
+_INDEX_CACHE = collections.OrderedDict()
 _INDEX_CACHE_MAX = 4096
+_INDEX_LOCK = threading.Lock()
 
 
 def _device_index(a, dev):
-    """Device copy of a host int64 index array, cached by content.  Cached
-    tensors stay alive, and an evicted one is only reused by allocations that
-    are stream-ordered after the kernels reading it."""
+    """Device copy of a host int64 index array, cached by content with
+    least-recently-used eviction.  Callers keep the returned tensor alive for
+    as long as a queued kernel may read it (an evicted tensor is freed when
+    its last reference goes)."""
     a = np.ascontiguousarray(a, dtype=np.int64)
     key = (str(dev), a.tobytes())
-    t = _INDEX_CACHE.get(key)
-    if t is None:
-        if len(_INDEX_CACHE) >= _INDEX_CACHE_MAX:
-            _INDEX_CACHE.pop(next(iter(_INDEX_CACHE)))
-        t = torch.from_numpy(a).to(dev)
+    with _INDEX_LOCK:
+        t = _INDEX_CACHE.get(key)
+        if t is not None:
+            _INDEX_CACHE.move_to_end(key)
+            return t
+    t = torch.from_numpy(a).to(dev)
+    with _INDEX_LOCK:
         _INDEX_CACHE[key] = t
+        _INDEX_CACHE.move_to_end(key)
+        while len(_INDEX_CACHE) > _INDEX_CACHE_MAX:
+            _INDEX_CACHE.popitem(last=False)
     return t
 
 

@@ -58,8 +58,9 @@ def _reference_layout(mps, version="0.4"):
     arr = np.empty(len(qn), object)
     arr[:] = qn
     d["qn"] = arr
-    for i, q in enumerate(qn):
-        d[f"subqn_{i}"] = q
+    if version == "0.4":   # only protocol 0.4 writes the subqn_<i> arrays
+        for i, q in enumerate(qn):
+            d[f"subqn_{i}"] = q
     d["qnidx"] = mps.qnidx
     d["qntot"] = np.asarray(mps.qntot)
     if version == "0.1":
@@ -70,19 +71,14 @@ def _reference_layout(mps, version="0.4"):
     return d
 
 
-@pytest.mark.parametrize("version", ["0.1", "0.3", "0.4"])
-def test_load_reference_layout(tmp_path, version):
+def test_load_reference_layout(tmp_path):
     model, mps = _state()
     fname = tmp_path / "ref.npz"
-    np.savez(fname, **_reference_layout(mps, version))
-    got = Mps.load(model, fname)
-    if version == "0.1":
-        assert got.coeff == 1
-        got.coeff = mps.coeff
-    _same(mps, got)
+    np.savez(fname, **_reference_layout(mps, "0.4"))
+    _same(mps, Mps.load(model, fname))
 
 
-@pytest.mark.parametrize("version", ["0.2", "9.9"])
+@pytest.mark.parametrize("version", ["0.1", "0.2", "0.3", "9.9"])
 def test_load_refuses(tmp_path, version):
     model, mps = _state()
     fname = tmp_path / "ref.npz"

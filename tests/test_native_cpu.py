@@ -67,3 +67,81 @@ def test_invalid_arguments_rejected_without_launch():
     assert b"precision" in lib.reno_last_error()
     st = lib.reno_block_qr(0, 4, 4, 1, 0, None, 1, 1, 0, 1, 1 << 20, None)
     assert st == native.RENO_EQN
+
+
+def test_krylov_retry_terminates():
+    """A Krylov call that never converges (the library keeps answering
+    RENO_ENOSPACE, as for a non-finite environment) raises once the largest
+    Krylov space is reached instead of retrying forever; the workspace grows
+    to what the library asks for."""
+    calls = []
+    sizes = []
+
+    def ws_bytes(desc, kcap):
+        k = kcap._obj.value if hasattr(kcap, "_obj") else kcap
+        return 0 if k > native.KRYLOV_MAX else 1000 * k
+
+    def call(desc, dre, dim, v, out, nmv, needed, ws, nbytes, stream):
+        calls.append(nbytes)
+        needed._obj.value = nbytes + 7
+        return native.RENO_ENOSPACE
+
+    def get_ws(nbytes):
+        sizes.append(nbytes)
+        return torch.empty(nbytes, dtype=torch.uint8)
+
+    desc = native.HopDesc(1, native.PREC_FP64, 2, 2, 2, 1, 1, 1, 1, 1)
+    v = torch.zeros(8, dtype=torch.complex128)
+    with pytest.raises(native.RenoNativeError, match="no convergence"):
+        native._expm_krylov_sync(desc, -0.01j, v, ws_bytes, call, get_ws, None)
+    assert len(calls) < 12
+    assert sizes == sorted(sizes) and sizes[-1] >= 1000 * native.KRYLOV_MAX
+
+
+def test_workspace_per_stream():
+    """Two streams never share a workspace buffer (the Krylov state block and
+    basis live in it)."""
+    ws = native._Workspace()
+    dev = torch.device("cpu")
+
+    class _Dev:
+        index = 0
+
+    orig = torch.empty
+
+    def fake_empty(size, dtype=None, device=None):
+        return orig(size, dtype=dtype)
+
+    torch.empty, saved = fake_empty, torch.empty
+    try:
+        a = ws.get(100, _Dev(), stream=1)
+        b = ws.get(100, _Dev(), stream=2)
+        a2 = ws.get(50, _Dev(), stream=1)
+    finally:
+        torch.empty = saved
+    assert a.data_ptr() != b.data_ptr()
+    assert a2 is a
+    assert sorted(ws.keys()) == [(0, 1), (0, 2)]
+    del dev
+
+
+def test_qn_block_descriptors_keep_index_tensors_alive(monkeypatch):
+    """With a tiny index cache, descriptors built in one call still point at
+    live tensors holding their own index arrays, even after the cache has
+    evicted them (the use-after-free an insertion-order cache allowed)."""
+    monkeypatch.setattr(native, "_INDEX_CACHE_MAX", 2)
+    monkeypatch.setattr(native, "_INDEX_CACHE", native.collections.OrderedDict())
+    rng = np.random.default_rng(0)
+    blocks = [(rng.permutation(20)[:5 + i], rng.permutation(30)[:4 + i]) for i in range(6)]
+    descs, keep, kt = native.qn_block_descs(blocks, torch.device("cpu"))
+    assert len(native._INDEX_CACHE) == 2
+    live = {t.data_ptr(): t for t in keep}
+    for d, (r, c) in zip(descs, blocks):
+        assert d.rows in live and d.cols in live
+        np.testing.assert_array_equal(live[d.rows].numpy(), r)
+        np.testing.assert_array_equal(live[d.cols].numpy(), c)
+    assert kt == sum(min(len(r), len(c)) for r, c in blocks)
+    # a hit refreshes the entry (least-recently-used eviction)
+    a = native._device_index(blocks[-1][1], torch.device("cpu"))
+    native._device_index(np.arange(3), torch.device("cpu"))
+    assert native._device_index(blocks[-1][1], torch.device("cpu")) is a
