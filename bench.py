@@ -9,12 +9,13 @@ with coefficient 1e-16 (the reference's SpinBosonDynamics.init_mps), built
 on the host before timing.  One step = Mps.evolve = two half-sweeps.
 
 Both device precision modes run from the same initial MPS: "fp64"
-(complex128 everywhere) and "mixed" (complex64 environments + FP32 matrix
-cores, FP64 Krylov vectors / reductions / exponential / QR).  ``value`` is
-the rate of the faster mode whose largest <sigma_z> difference from the FP64
-trajectory stays within 1e-6 (the FP64 mode itself when the mixed mode
-misses that bound); both rates, their ratio and the difference are reported
-beside it.
+(complex128 everywhere) and "mixed" (FP64 environments, vectors, reductions,
+exponential and QR; each Lanczos iteration's matvec on FP32 matrix cores
+once the step's result weighs the new Lanczos vector below 1e-2).  ``value``
+is the rate of the faster mode whose largest <sigma_z> difference from the
+FP64 trajectory stays within 1e-6 (the FP64 mode itself when the mixed mode
+misses that bound); both rates, their ratio, the difference and the matvec
+counts are reported beside it.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -87,6 +88,7 @@ def run_mode(mps0, mpo, precision, dt, warmup, steps, profile):
     redo1 = Mps.krylov_redo_count
     seconds = 0.0
     matvecs = []
+    low = []
     for k in range(steps):
         if profile:
             # a new record for the first step, resumed for the later ones
@@ -99,6 +101,7 @@ def run_mode(mps0, mpo, precision, dt, warmup, steps, profile):
         if profile:
             native.prof_enable(0)
         matvecs.append(sum(mps.evolve_config.local_steps))
+        low.append(getattr(mps.evolve_config, "fp32_matvecs", 0))
         sz.append(sigma_z(mps))
     prof = None
     if profile:
@@ -111,7 +114,7 @@ def run_mode(mps0, mpo, precision, dt, warmup, steps, profile):
     log(f"{precision}: steps redone after an exhausted Krylov budget: warmup {redo1 - redo0}, "
         f"timed {redone}")
     return {"seconds": seconds, "steps_per_s": steps / seconds, "sz": sz, "prof": prof,
-            "matvecs_per_step": matvecs, "final": mps, "environ": env, "redone": redone}
+            "matvecs_per_step": matvecs, "fp32_matvecs_per_step": low, "final": mps, "environ": env, "redone": redone}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tdvp_c4_m256_fp64_hbm_traffic.md")
@@ -350,6 +353,11 @@ def main():
         out["mixed_over_fp64"] = round(res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"], 3)
         out["max_abs_sigma_z_err_mixed_vs_fp64"] = err
         out["mixed_within_bound"] = err <= ACCURACY_BOUND
+        out["steps_compared"] = len(res["fp64"]["sz"]) - 1
+        mv64, mvmx = sum(res["fp64"]["matvecs_per_step"]), sum(res["mixed"]["matvecs_per_step"])
+        out["matvecs_fp64"] = mv64
+        out["matvecs_mixed"] = mvmx
+        out["mixed_fp32_matvec_share"] = round(sum(res["mixed"]["fp32_matvecs_per_step"]) / mvmx, 3)
         other = "mixed" if main_mode == "fp64" else "fp64"
         out[other + "_roofline"] = roofline(res[other]["prof"], res[other]["seconds"], a.steps)
     if not a.no_cpu:

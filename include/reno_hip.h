@@ -128,6 +128,29 @@ int reno_expm_krylov_async(const reno_hop_desc* op, double dt_re, double dt_im,
                            const void* v0, void* out, int max_matvec, int* info,
                            void* ws, size_t ws_bytes, void* stream);
 
+/* ---- expm_krylov with adaptive precision (the mixed mode) --------------
+ * The same computation as reno_expm_krylov_async (reference krylov.py:27-84,
+ * stopping rule included), but Lanczos iteration j applies op (FP64:
+ * complex128 L/R, float64 W) for j < j_switch and op_lo (the same operator
+ * held in RENO_PREC_MIXED: complex64 L/R, float32 W, FP32 matrix cores) for
+ * j >= j_switch.  Vectors, recurrence, reductions, the tridiagonal
+ * exponential and the stopping rule stay FP64.  A matvec whose relative
+ * error is delta perturbs the result by about |c_{j-1}| delta, c_k the
+ * result's coefficient of the k-th Lanczos vector, so the low-precision
+ * tail costs nothing in accuracy once those coefficients are small.
+ * info is a device int[3]: info[0..1] as reno_expm_krylov_async, info[2] =
+ * the switch point measured on this call's result with threshold tau (the
+ * first j >= 2 with |c_k| < tau |v0| for every k >= j - 1; 1 << 30 if the
+ * last coefficient is not below tau), which the caller hands to the same
+ * local step of the next time step.  j_switch >= 1 << 30 gives the FP64
+ * path exactly.  Workspace: reno_krylov_workspace_bytes(op, max_matvec).
+ * No reference counterpart: the reference's only low-precision knob is the
+ * global complex64 switch (utils/backend.py:111-114). */
+int reno_expm_krylov_mixed_async(const reno_hop_desc* op, const reno_hop_desc* op_lo,
+                                 int j_switch, double tau, double dt_re, double dt_im,
+                                 const void* v0, void* out, int max_matvec, int* info,
+                                 void* ws, size_t ws_bytes, void* stream);
+
 /* ---- svd_qn QR / RQ branch (replaces svd_qn.py:99-240, QR=True,
  * full_matrices=False) ------------------------------------------------
  * coef is the complex128 (rows x cols) coefficient matrix.  For each block

@@ -584,9 +584,42 @@ __global__ void k_info(const int* st, int queued, int* info) {
   info[1] = kind == 0 ? 0 : (kind < 0 ? -1 : 1);
 }
 
+// Precision switch point of the mixed entry point, measured on this call's
+// result y = sum_k c_k v_k (v_k normalised): the smallest j >= 2 such that
+// every coefficient from c_{j-1} on has |c_k| < tau |v0|.  A matvec H v_j
+// whose error is delta contributes about |c_{j-1}| delta to the result, so
+// from that iteration on the FP32 product is as good as the FP64 one at
+// the tolerance tau * 1e-7.  info[2] = 1 << 30 when the last coefficient is
+// not below tau (no iteration of this call would have been low precision).
+__global__ void k_switch_point(const int* st, const double2* __restrict__ coef_all, int kcap,
+                               const double* __restrict__ scal, double tau, int* info) {
+  const int kind = final_kind(st);
+  int m = 0;
+  const double2* c = nullptr;
+  if (kind == 1) {
+    m = st[KS_CONV];
+    c = coef_all + (size_t)st[KS_RESIDX] * kcap;
+  } else if (kind == 2 || kind == 3) {
+    m = st[KS_MFINAL];
+    c = coef_all + (size_t)2 * kcap;
+  }
+  int js = 1 << 30;
+  if (c && m > 0) {
+    const double thr = tau * scal[0];
+    int last_big = -1;
+    for (int k = m - 1; k >= 0; --k)
+      if (!(sqrt(c[k].x * c[k].x + c[k].y * c[k].y) < thr)) {
+        last_big = k;
+        break;
+      }
+    if (last_big < m - 1) js = max(2, last_big + 2);
+  }
+  info[2] = js;
+}
+
 struct KrylovLayout {
-  size_t off_wl, off_t1, off_t2, off_split, off_V, off_w, off_r0, off_r1, off_part, off_part2,
-      off_scal, off_coef;
+  size_t off_wl, off_wl_lo, off_t1, off_t2, off_split, off_V, off_w, off_r0, off_r1, off_part,
+      off_part2, off_scal, off_coef;
   size_t split_bytes;
   size_t total;
 };
@@ -598,6 +631,11 @@ static KrylovLayout krylov_layout(const reno_hop_desc* op, int kcap) {
   const size_t vb = align_up((size_t)n * 16);
   size_t o = 0;
   k.off_wl = o; o += h.wl_bytes;
+  // the FP32 copy of the permuted MPO for the low-precision iterations of
+  // the mixed entry point (a few KB)
+  k.off_wl_lo = o;
+  if (op->nsite == 1)
+    o += align_up((size_t)op->Dl * op->d * op->d * op->Dr * sizeof(float));
   k.off_t1 = o; o += h.t1_bytes;
   k.off_t2 = o; o += h.t2_bytes;
   k.off_split = o; o += h.part_bytes;
@@ -629,6 +667,11 @@ static thread_local HostStage t_stage;
 // One call's device buffers and launch parameters.
 struct KrylovRun {
   const reno_hop_desc* op;
+  // mixed entry point: iterations j >= j_switch apply op_lo (complex64
+  // environments, float32 MPO, FP32 matrix cores) instead of op
+  const reno_hop_desc* op_lo = nullptr;
+  int64_t j_switch = INT64_MAX;
+  void* wl_lo = nullptr;
   hipStream_t s;
   int64_t n;
   int kcap;
@@ -665,6 +708,7 @@ static void bind_run(KrylovRun& r, const reno_hop_desc* op, hipStream_t s, void*
   r.L = krylov_layout(op, kcap);
   char* base = (char*)ws;
   r.wl = base + r.L.off_wl;
+  r.wl_lo = base + r.L.off_wl_lo;
   r.t1 = base + r.L.off_t1;
   r.t2 = base + r.L.off_t2;
   r.split = base + r.L.off_split;
@@ -692,6 +736,9 @@ static void bind_run(KrylovRun& r, const reno_hop_desc* op, hipStream_t s, void*
 static int queue_start(KrylovRun& r, const void* v0) {
   if (r.op->nsite == 1)
     RENO_TRY(permute_w(r.op->prec, r.op->W, r.wl, r.op->Dl, r.op->d, r.op->Dr, r.s));
+  if (r.op_lo && r.op_lo->nsite == 1)
+    RENO_TRY(permute_w(r.op_lo->prec, r.op_lo->W, r.wl_lo, r.op_lo->Dl, r.op_lo->d, r.op_lo->Dr,
+                       r.s));
   hipLaunchKernelGGL(k_norm2, dim3(r.np), dim3(kRedThreads), 0, r.s, (const double2*)v0, r.n,
                      r.part);
   hipLaunchKernelGGL(k_kry_init, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part, r.np,
@@ -727,8 +774,9 @@ static int queue_iter(KrylovRun& r, int64_t j) {
   const int* stop = r.st + KS_STOP;
   double2* vj = r.V + (size_t)j * r.vstride;
   // matvec with the dot partials of Re<w, V[j]> fused into its last stage
-  RENO_TRY(hop_apply_core(r.op, r.wl, vj, r.w, r.t1, r.t2, r.s, stop, r.split, r.L.split_bytes,
-                          vj, r.part, r.np));
+  const bool lo = r.op_lo && j >= r.j_switch;
+  RENO_TRY(hop_apply_core(lo ? r.op_lo : r.op, lo ? r.wl_lo : r.wl, vj, r.w, r.t1, r.t2, r.s, stop,
+                          r.split, r.L.split_bytes, vj, r.part, r.np));
   if (j == r.n - 1) {
     hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(kRedThreads), 0, r.s, (const double*)r.part,
                        (const double*)(r.part2 + (size_t)(j & 1) * kMaxPartials), r.np, r.d_alpha,
@@ -875,6 +923,44 @@ extern "C" int reno_expm_krylov_async(const reno_hop_desc* op, double dt_re, dou
   for (int64_t j = 0; j < jmax; ++j) RENO_TRY(queue_iter(r, j));
   RENO_TRY(queue_final(r, out, (int)jmax));
   hipLaunchKernelGGL(k_info, dim3(1), dim3(1), 0, s, (const int*)r.st, (int)jmax, info);
+  RENO_CHECK_LAUNCH();
+  return RENO_OK;
+}
+
+extern "C" int reno_expm_krylov_mixed_async(const reno_hop_desc* op, const reno_hop_desc* op_lo,
+                                            int j_switch, double tau, double dt_re, double dt_im,
+                                            const void* v0, void* out, int max_matvec, int* info,
+                                            void* ws, size_t ws_bytes, void* stream) {
+  RENO_TRY(check_args(op, v0, out));
+  RENO_TRY(check_hop(op_lo));
+  if (op->prec != RENO_PREC_FP64 || op_lo->prec != RENO_PREC_MIXED || op->nsite != op_lo->nsite ||
+      op->Ml != op_lo->Ml || op->d != op_lo->d || op->Mr != op_lo->Mr || op->Dl != op_lo->Dl ||
+      op->Dr != op_lo->Dr) {
+    set_error("expm_krylov_mixed: op must be FP64 and op_lo the same operator in the mixed mode");
+    return RENO_EINVAL;
+  }
+  if (!info || max_matvec < 1 || !(tau >= 0.0)) {
+    set_error("expm_krylov_mixed: null info, max_matvec < 1 or negative tau");
+    return RENO_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int kcap = kcap_for(op, ws_bytes);
+  const int64_t jmax = std::min<int64_t>(hop_vec_len(op), max_matvec);
+  if (!ws || krylov_layout(op, kcap).total > ws_bytes || jmax > kcap) {
+    set_error("expm_krylov_mixed: workspace %zu holds %d Krylov vectors, %lld requested",
+              ws_bytes, kcap, (long long)jmax);
+    return RENO_ENOSPACE;
+  }
+  KrylovRun r;
+  bind_run(r, op, s, ws, kcap, dt_re, dt_im);
+  r.op_lo = op_lo;
+  r.j_switch = std::max(j_switch, 1);
+  RENO_TRY(queue_start(r, v0));
+  for (int64_t j = 0; j < jmax; ++j) RENO_TRY(queue_iter(r, j));
+  RENO_TRY(queue_final(r, out, (int)jmax));
+  hipLaunchKernelGGL(k_info, dim3(1), dim3(1), 0, s, (const int*)r.st, (int)jmax, info);
+  hipLaunchKernelGGL(k_switch_point, dim3(1), dim3(1), 0, s, (const int*)r.st,
+                     (const double2*)r.d_coef, r.kcap, (const double*)r.scal, tau, info);
   RENO_CHECK_LAUNCH();
   return RENO_OK;
 }

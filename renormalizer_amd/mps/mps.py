@@ -37,7 +37,11 @@ from renormalizer_amd.utils import (CompressConfig, EvolveConfig, EvolveMethod, 
 
 logger = logging.getLogger(__name__)
 
-_PREC = {"fp64": native.PREC_FP64, "mixed": native.PREC_MIXED}
+# environment / operator precision of each mode: "mixed" keeps FP64
+# environments and switches only the tail of each Lanczos iteration to FP32
+# (native.expm_krylov_mixed_async); "fp32" runs every contraction on FP32
+# matrix cores with complex64 environments
+_PREC = {"fp64": native.PREC_FP64, "mixed": native.PREC_FP64, "fp32": native.PREC_MIXED}
 
 
 def _min_abs(t1, t2):
@@ -89,6 +93,14 @@ def _describe(local_steps):
     with warnings.catch_warnings():
         warnings.simplefilter("ignore", RuntimeWarning)
         return stats.describe(local_steps)
+
+def _fp32_matvecs(counts, switches):
+    """Matvecs of a step that ran on the FP32 operator: per Krylov call, the
+    iterations from its switch point on."""
+    if not switches:
+        return 0
+    return int(sum(max(0, c - max(1, switches.get(k, native.SWITCH_NEVER))) for k, c in counts))
+
 
 class Mps(MatrixProduct):
     def __init__(self):
@@ -372,17 +384,22 @@ class Mps(MatrixProduct):
             # counts of a step run with host waits (the first one, or a redo)
             # move more than those of consecutive asynchronous steps
             wide = cache[3][2]
-        mps, environ, counts = self._tdvp_ps_sweep(mpo, evolve_dt, prec, cache[2] if reuse else None,
-                                                   budgets, wide)
+        switches = cache[3][3] if budgets is not None else None
+        adaptive = self.evolve_config.precision == "mixed"
+        mps, environ, counts, switch = self._tdvp_ps_sweep(
+            mpo, evolve_dt, prec, cache[2] if reuse else None, budgets, wide,
+            switches if adaptive else None, self.evolve_config.mixed_tau if adaptive else None)
         synchronous = budgets is None
         if counts is None:
             # an iteration budget was too small: redo the step with host waits
-            mps, environ, counts = self._tdvp_ps_sweep(mpo, evolve_dt, prec, None, None)
+            mps, environ, counts, switch = self._tdvp_ps_sweep(mpo, evolve_dt, prec, None, None)
             synchronous = True
         local_steps = [c for _, c in counts]
         mps.evolve_config.stat = _describe(local_steps)
         mps.evolve_config.local_steps = local_steps
-        mps._env_cache = (mpo, prec, environ, (complex(evolve_dt), dict(counts), synchronous))
+        mps.evolve_config.fp32_matvecs = _fp32_matvecs(counts, switches if adaptive and not synchronous
+                                                       else None)
+        mps._env_cache = (mpo, prec, environ, (complex(evolve_dt), dict(counts), synchronous, switch))
         return mps
 
     @adaptive_tdvp
@@ -470,11 +487,16 @@ class Mps(MatrixProduct):
     # Krylov budget proved too small (diagnostics)
     krylov_redo_count = 0
 
-    def _tdvp_ps_sweep(self, mpo, evolve_dt, prec, environ, budgets, wide=True):
+    def _tdvp_ps_sweep(self, mpo, evolve_dt, prec, environ, budgets, wide=True, switches=None,
+                       tau=None):
         """The sweep itself.  With ``budgets`` (matvec counts of the previous
         step keyed by position) every Krylov call is queued asynchronously and
         the counts are read once at the end; returns counts None when a
-        budget ran out."""
+        budget ran out.  With ``tau`` (the mixed mode, asynchronous steps
+        only) each Krylov call switches to FP32 matvecs at the iteration
+        ``switches`` holds for its position (all FP64 where it holds none)
+        and measures the switch point for the next step; returns (mps,
+        environ, counts, measured switch points or None)."""
         mps = self.to_complex()
         if not mps.on_device:
             mps.to_device()
@@ -485,8 +507,11 @@ class Mps(MatrixProduct):
             # only the left environments are read before being rebuilt
             environ = Environ(mps._mp, wt, "L" if not mps.to_right else "R", prec=prec)
         use_async = budgets is not None
+        adaptive = use_async and tau is not None
+        if adaptive:
+            wt32 = mpo.device_tensors(native.PREC_MIXED, dev)
         if use_async:
-            info = torch.zeros((4 * n + 4, 2), dtype=torch.int32, device=dev)
+            info = torch.zeros((4 * n + 4, 3), dtype=torch.int32, device=dev)
         keys = []
         counts = []
 
@@ -502,7 +527,14 @@ class Mps(MatrixProduct):
             prev = budgets.get(key, 0)
             budget = prev + (prev // 4 + 8 if wide else prev // 16 + 6)
             budget = max(budget, 12)
-            out = native.expm_krylov_async(l_array, w, r_array, v, dt, prec, budget, info[len(keys)])
+            if adaptive:
+                js = switches.get(key, native.SWITCH_NEVER) if switches else native.SWITCH_NEVER
+                w32 = None if w is None else wt32[key[1]]
+                out = native.expm_krylov_mixed_async(
+                    l_array, w, r_array, l_array.to(torch.complex64), w32, r_array.to(torch.complex64),
+                    v, dt, budget, js, tau, info[len(keys)])
+            else:
+                out = native.expm_krylov_async(l_array, w, r_array, v, dt, prec, budget, info[len(keys)])
             keys.append(key)
             return out
 
@@ -552,11 +584,13 @@ class Mps(MatrixProduct):
                 Mps.krylov_redo_count += 1
                 logger.info("Krylov budget exhausted at %s; redoing the step with host waits",
                             [k for k, g in zip(keys, got) if g[1] == 0][:4])
-                return mps, environ, None
+                return mps, environ, None, None
             if (got[:, 1] < 0).any():
                 raise AssertionError("zero vector handed to expm_krylov (krylov.py:44)")
             counts = [(k, int(c)) for k, c in zip(keys, got[:, 0])]
-        return mps, environ, counts
+            if adaptive:
+                return mps, environ, counts, {k: int(x) for k, x in zip(keys, got[:, 2])}
+        return mps, environ, counts, None
 
     # ---- observables --------------------------------------------------------
     def expectation(self, mpo: Union[Mpo, Op, OpSum], self_conj: "Mps" = None):

@@ -33,7 +33,7 @@ EXPORTS = (
     "reno_hop_workspace_bytes", "reno_krylov_workspace_bytes",
     "reno_env_workspace_bytes", "reno_qr_workspace_bytes",
     "reno_hop_apply", "reno_env_update", "reno_env_close", "reno_expm_krylov",
-    "reno_expm_krylov_async", "reno_block_qr", "reno_heff_diag", "reno_heff_dense", "reno_zgemm",
+    "reno_expm_krylov_async", "reno_expm_krylov_mixed_async", "reno_block_qr", "reno_heff_diag", "reno_heff_dense", "reno_zgemm",
     "reno_probe_mfma16", "reno_prof_enable", "reno_prof_collect", "reno_prof_kind_info",
     "reno_prof_cancelled",
 )
@@ -110,6 +110,10 @@ def load_library(path: str = LIB_PATH):
         lib.reno_expm_krylov_async.restype = c_int
         lib.reno_expm_krylov_async.argtypes = [ctypes.POINTER(HopDesc), c_double, c_double, vp, vp,
                                                c_int, vp, vp, c_size, vp]
+        lib.reno_expm_krylov_mixed_async.restype = c_int
+        lib.reno_expm_krylov_mixed_async.argtypes = [ctypes.POINTER(HopDesc), ctypes.POINTER(HopDesc),
+                                                     c_int, c_double, c_double, c_double, vp, vp,
+                                                     c_int, vp, vp, c_size, vp]
         lib.reno_heff_diag.restype = c_int
         lib.reno_heff_diag.argtypes = [c_int] * 5 + [vp, vp, vp, vp, vp]
         lib.reno_heff_dense.restype = c_int
@@ -380,6 +384,37 @@ def expm_krylov_async(L, W, R, v, dt, prec, max_matvec, info):
     check(l.reno_expm_krylov_async(ctypes.byref(desc), dt.real, dt.imag, _ptr(v), _ptr(out),
                                    max_matvec, _ptr(info), _ptr(ws), ws.numel(), _stream()),
           "reno_expm_krylov_async")
+    return out
+
+
+# switch point meaning "every iteration in FP64"
+SWITCH_NEVER = 1 << 30
+
+
+def expm_krylov_mixed_async(L, W, R, L32, W32, R32, v, dt, max_matvec, j_switch, tau, info):
+    """Queue expm(dt * H_eff) v with adaptive precision, without waiting.
+
+    Lanczos iterations j < ``j_switch`` apply the FP64 operator (L, W, R:
+    complex128 / float64), later ones the same operator held in FP32
+    (L32, W32, R32: complex64 / float32).  ``info`` is a device int32 tensor
+    of three elements: (matvecs used, flag) as for ``expm_krylov_async``, and
+    the switch point measured on this call's result with threshold ``tau``
+    (``SWITCH_NEVER`` when no iteration qualifies)."""
+    desc = _krylov_desc(L, W, R, v, PREC_FP64)
+    desc_lo = _krylov_desc(L32, W32, R32, v, PREC_MIXED)
+    if info.dtype != torch.int32 or info.numel() < 3 or not info.is_cuda:
+        raise ValueError("info must be a device int32 tensor of three elements")
+    max_matvec = int(max(2, min(max_matvec, KRYLOV_ASYNC_MAX)))
+    dt = complex(dt)
+    l = lib()
+    out = torch.empty_like(v)
+    nbytes = l.reno_krylov_workspace_bytes(ctypes.byref(desc), max_matvec)
+    ws = workspace.get(nbytes, v.device)
+    check(l.reno_expm_krylov_mixed_async(ctypes.byref(desc), ctypes.byref(desc_lo),
+                                         int(min(j_switch, SWITCH_NEVER)), float(tau),
+                                         dt.real, dt.imag, _ptr(v), _ptr(out), max_matvec,
+                                         _ptr(info), _ptr(ws), ws.numel(), _stream()),
+          "reno_expm_krylov_mixed_async")
     return out
 
 

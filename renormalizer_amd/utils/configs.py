@@ -2,9 +2,13 @@
 
 Names, defaults and semantics follow the reference.  One field is new:
 ``EvolveConfig.precision`` selects the device arithmetic of the TDVP-PS
-sweep ("fp64": complex128 everywhere; "mixed": complex64 environments and
-FP32 matrix-core contractions with complex128 Krylov vectors, FP64
-reductions, tridiagonal exponential and QR).
+sweep: "fp64" (complex128 everywhere); "mixed" (complex128 environments,
+QR, Krylov vectors, reductions and tridiagonal exponential; each Lanczos
+iteration's matvec on FP64 matrix cores while the step's result still
+weighs the Lanczos vectors it creates by at least ``mixed_tau``, on FP32
+matrix cores with complex64 copies of the environments after that);
+"fp32" (every contraction on FP32 matrix cores with complex64
+environments; kept for comparison, it misses the 1e-6 population bound).
 """
 from enum import Enum
 from typing import Union
@@ -150,7 +154,7 @@ class EvolveMethod(Enum):
     tdvp_mu_vmf = "TDVP Matrix Unfolding Variable Mean Field"
 
 
-PRECISIONS = ("fp64", "mixed")
+PRECISIONS = ("fp64", "mixed", "fp32")
 
 
 class EvolveConfig:
@@ -160,7 +164,7 @@ class EvolveConfig:
                  adaptive=False, guess_dt=1e-1, adaptive_rtol=5e-4, taylor_order: int = None,
                  rk_solver="C_RK4", reg_epsilon=1e-10, ivp_rtol=1e-5, ivp_atol=1e-8,
                  ivp_solver="krylov", force_ovlp=True, precision: str = "fp64",
-                 krylov_async: bool = True):
+                 krylov_async: bool = True, mixed_tau: float = 1e-2):
         if isinstance(method, str):
             method = getattr(EvolveMethod, method)
         self.method = method
@@ -186,6 +190,10 @@ class EvolveConfig:
         # without waiting for the device (iteration budgets from the previous
         # step; the step is redone with waits when a budget proves too small)
         self.krylov_async = bool(krylov_async)
+        # mixed mode: a Lanczos iteration runs its matvec in FP32 once the
+        # coefficient of the previous Lanczos vector in the step's result is
+        # below mixed_tau times the norm of the starting vector
+        self.mixed_tau = float(mixed_tau)
         self.stat = None
 
     @property

@@ -1,0 +1,95 @@
+"""The metric workload (C4): Ohmic spin-boson model, 1 spin + 64 modes x 10
+levels, alpha = 0.3, omega_c = 10, Wang-1 grid, dt = 0.025, initial state
+spin up x vacuum expanded along H with coefficient 1e-16 (reference
+sbm/sbm.py:55-59), built exactly as bench.py builds it.
+
+* M = 128: two FP64 device steps against the numpy oracle on identical
+  tensors (reference mps/mps.py:1267-1404): <sigma_z> to 1e-8, every site
+  tensor to 1e-8 relative, Lanczos matvec counts, quantum numbers.
+* M = 256: ten device steps of each mode.  FP64: |norm - 1| <= 1e-12 and
+  the energy <H> constant to the Krylov tolerance.  Mixed (FP32 matvecs in
+  the tail of every Lanczos iteration): <sigma_z> within 1e-8 of the FP64
+  run, matvec counts within 2 % of FP64's on every step.
+"""
+import numpy as np
+import pytest
+
+from oracle import tdvp as orc
+from renormalizer_amd import EvolveConfig, EvolveMethod
+from tests import helpers
+from tests.test_tdvp_gpu import _assert_steps_match, _sz_dev, _sz_orc
+
+pytestmark = pytest.mark.gpu
+
+DT = 0.025
+
+
+def _workload(m):
+    import bench
+    _, mpo, mps = bench.build_workload(64, 10, m)
+    return mps, mpo
+
+
+@pytest.fixture(scope="module")
+def c4_256():
+    print("[C4 M=256: host initial state]", flush=True)
+    return _workload(256)
+
+
+def test_c4_m128_fp64_oracle_parity():
+    print("[C4 M=128: initial state, 2 device steps vs oracle]", flush=True)
+    mps, mpo = _workload(128)
+    assert max(mps.bond_dims) == 128
+    st = helpers.to_state(mps)
+    w = helpers.mpo_arrays(mpo)
+    dev = mps.copy()
+    dev.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps)
+    dev.to_device()
+    sd, so = [], []
+    for k in range(2):
+        dev = dev.evolve(mpo, DT)
+        st, ls = orc.tdvp_ps(st, w, DT)
+        sd.append(list(dev.evolve_config.local_steps))
+        so.append(ls)
+        print(f"[C4 M=128 step {k + 1}] sz dev {_sz_dev(dev):.12f} oracle {_sz_orc(st):.12f}", flush=True)
+        assert abs(_sz_dev(dev) - _sz_orc(st)) < 1e-8
+    _assert_steps_match(sd, so)
+    for a, b in zip(dev.numpy_sites(), st.sites):
+        assert np.linalg.norm(a - b) <= 1e-8 * np.linalg.norm(b)
+    assert [np.array(q).tolist() for q in dev.qn] == [np.array(q).tolist() for q in st.qn]
+    assert dev.qnidx == st.qnidx and dev.to_right == st.to_right
+
+
+def _energy(mps, mpo):
+    return float(np.real(mps.expectation(mpo)))
+
+
+def test_c4_m256_fp64_and_mixed(c4_256):
+    mps0, mpo = c4_256
+    assert max(mps0.bond_dims) == 256
+    runs = {}
+    for mode in ("fp64", "mixed"):
+        print(f"[C4 M=256 {mode}: 10 steps]", flush=True)
+        m = mps0.copy()
+        m.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps, precision=mode)
+        m.to_device()
+        e0 = _energy(m, mpo)
+        sz, steps, low, norms, energies = [_sz_dev(m)], [], [], [], []
+        for _ in range(10):
+            m = m.evolve(mpo, DT)
+            sz.append(_sz_dev(m))
+            steps.append(sum(m.evolve_config.local_steps))
+            low.append(m.evolve_config.fp32_matvecs)
+            norms.append(m.mp_norm)
+            energies.append(_energy(m, mpo))
+        drift = max(abs(e - e0) for e in energies) / abs(e0)
+        print(f"[C4 M=256 {mode}] matvecs {steps} fp32 {low} |norm-1| "
+              f"{max(abs(x - 1) for x in norms):.1e} energy {e0:.10f} drift {drift:.2e}", flush=True)
+        assert max(abs(x - 1) for x in norms) <= 1e-12
+        assert drift <= 1e-8
+        runs[mode] = (np.array(sz), np.array(steps), np.array(low))
+    (s64, n64, _), (smx, nmx, low) = runs["fp64"], runs["mixed"]
+    print(f"[C4 M=256 mixed vs fp64] max |dsz| {np.abs(smx - s64).max():.2e}", flush=True)
+    assert np.abs(smx - s64).max() <= 1e-8
+    np.testing.assert_allclose(nmx, n64, rtol=0.02)
+    assert low[2:].sum() > 0.2 * nmx[2:].sum()
