@@ -161,48 +161,120 @@ def roofline(prof, seconds, steps):
     }
 
 
-def cpu_baseline(mps_dev, env, mpo, dt, sample_reps, gemm_flop_per_step):
-    """Time the numpy oracle on full-size (256, 10, 256) site updates taken
-    from the evolved state and scale to one time step by algorithmic FLOP."""
+def host_info():
+    """CPU model, usable cores, BLAS threads, numpy/scipy versions and BLAS
+    library of this host."""
+    import numpy as np
+    import scipy
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = "unknown"
+    try:
+        cfg = np.show_config(mode="dicts")
+        b = cfg.get("Build Dependencies", {}).get("blas", {})
+        blas = f"{b.get('name', '?')} {b.get('version', '')}".strip()
+    except Exception:
+        pass
+    return {"cpu_model": model, "affinity_cores": _CORES,
+            "reno_num_threads": int(os.environ["RENO_NUM_THREADS"]),
+            "numpy": np.__version__, "scipy": scipy.__version__, "blas": blas}
+
+
+def qr_flop(m, n):
+    """Householder QR of an m x n complex matrix (m >= n) plus forming the
+    m x n Q: 2 x 4(2 m n^2 - 2 n^3 / 3) real FLOP."""
+    m, n = max(m, n), min(m, n)
+    return 2 * 4 * (2 * m * n * n - 2 * n ** 3 / 3)
+
+
+def site_update_flop(ml, d, mr, dl, dr, k1, k0, mb=None):
+    """Algorithmic FLOP of one right-to-left site update: k1 one-site
+    matvecs, the RQ of the (ml) x (d mr) site matrix with its Q, the
+    environment update, k0 bond matvecs (bond mb x ml) and the bond absorb."""
+    mb = ml if mb is None else mb
+    f_hop1 = 8 * (ml * ml * dl * d * mr + ml * d * dr * mr * mr) + 4 * ml * mr * dl * dr * d * d
+    f_hop0 = 8 * dl * mb * ml * (mb + ml)
+    f_absorb = 8 * mb * ml * ml * d
+    return k1 * f_hop1 + qr_flop(d * mr, ml) + f_hop1 + k0 * f_hop0 + f_absorb
+
+
+def cpu_baseline(mps_dev, env, mpo, dt, sample_reps, flop_per_step):
+    """The numpy oracle timed on this host on a bounded sample of the
+    workload: ``sample_reps`` consecutive right-to-left site updates of the
+    evolved state's full-size middle sites (Lanczos exponential of the
+    one-site operator, RQ, environment update, bond exponential, bond
+    absorb, exactly as one step does them), converted to time steps per
+    second by the algorithmic FLOP of the sample (QR included) against the
+    FLOP of one device step."""
     import numpy as np
 
     from oracle import tdvp as orc
     n = len(mps_dev)
-    site = n // 2
     sites = mps_dev.numpy_sites()
     w = [np.asarray(x) for x in mpo]
-    L = env.read("L", site - 1).cpu().numpy().astype(np.complex128)
+    site = n // 2 + sample_reps // 2
     R = env.read("R", site + 1).cpu().numpy().astype(np.complex128)
-    c = sites[site]
-    Ml, d, Mr = c.shape
-    Dl, Dr = w[site].shape[0], w[site].shape[3]
-    t0 = time.perf_counter()
+    t = 0.0
     flop = 0.0
-    for _ in range(sample_reps):
-        v, k = orc.expm_krylov(lambda y: orc.hop1(L, w[site], R, y.reshape(c.shape)).ravel(),
+    for rep in range(sample_reps):
+        i = site - rep
+        L = env.read("L", i - 1).cpu().numpy().astype(np.complex128)
+        t1 = time.perf_counter()
+        c = sites[i]
+        Ml, d, Mr = c.shape
+        Dl, Dr = w[i].shape[0], w[i].shape[3]
+        v, k = orc.expm_krylov(lambda y: orc.hop1(L, w[i], R, y.reshape(c.shape)).ravel(),
                                -1j * dt / 2, c.ravel())
         v = v.reshape(c.shape)
-        qnl = np.zeros((Ml, 1), dtype=int)
-        qnr = np.zeros((d * Mr, 1), dtype=int)
-        u, _, vv, _ = orc.svd_qn(v, qnl, qnr, np.zeros(1, dtype=int), QR=True, system="R",
-                                 full_matrices=False)
+        u, _, vv, _ = orc.svd_qn(v, np.zeros((Ml, 1), dtype=int), np.zeros((d * Mr, 1), dtype=int),
+                                 np.zeros(1, dtype=int), QR=True, system="R", full_matrices=False)
         vt = vv.T.reshape(-1, d, Mr)
-        r_new = orc.contract_one_site(R, vt, w[site], "R")
-        u_t, k0 = orc.expm_krylov(lambda y: orc.hop0(L, r_new, y.reshape(u.shape)).ravel(),
+        R = orc.contract_one_site(R, vt, w[i], "R")
+        u_t, k0 = orc.expm_krylov(lambda y: orc.hop0(L, R, y.reshape(u.shape)).ravel(),
                                   1j * dt / 2, u.ravel())
-        np.tensordot(sites[site - 1], u_t.reshape(u.shape), axes=(-1, 0))
-        f_hop1 = 8 * (Ml * Ml * Dl * d * Mr + Ml * d * Dr * Mr * Mr) + 4 * Ml * Mr * Dl * Dr * d * d
-        f_hop0 = 8 * Dl * Ml * u.shape[1] * (Ml + u.shape[1])
-        flop += k * f_hop1 + f_hop1 + k0 * f_hop0
-    t = time.perf_counter() - t0
+        sites[i - 1] = np.tensordot(sites[i - 1], u_t.reshape(u.shape), axes=(-1, 0))
+        t += time.perf_counter() - t1
+        flop += site_update_flop(Ml, d, Mr, Dl, Dr, k, k0, mb=u.shape[1])
     rate = flop / t
+    info = host_info()
     return {
-        "value": round(rate / gemm_flop_per_step, 5), "unit": "sweeps/s",
-        "cores": int(os.environ["RENO_NUM_THREADS"]), "kind": "port",
-        "sample": (f"{sample_reps} full-size site updates (256x10x256: Lanczos expm, RQ, environment "
-                   f"update, bond expm, absorb) of the evolved M=256 state by the numpy oracle in "
-                   f"{t:.1f} s = {rate / 1e9:.1f} GFLOP/s, scaled by the per-step algorithmic GEMM FLOP"),
+        "value": round(rate / flop_per_step, 5), "unit": "sweeps/s",
+        "cores": info["reno_num_threads"], "kind": "port",
+        "sample": (f"{sample_reps} consecutive right-to-left full-size site updates (256x10x256: "
+                   f"Lanczos expm, RQ, environment update, bond expm, bond absorb) of the evolved "
+                   f"M=256 state by the numpy oracle in {t:.1f} s = {rate / 1e9:.1f} GFLOP/s, "
+                   f"scaled by the algorithmic FLOP of one device step ({flop_per_step / 1e12:.2f} "
+                   f"TFLOP, QR included)"),
+        "host": info,
     }
+
+
+def step_flop(counts, shapes, mpo_shapes):
+    """Algorithmic FLOP of one time step, counted like site_update_flop,
+    from the executed matvec counts of its Krylov calls (keys (half, site,
+    1) one-site, (half, site, 0) bond; half 0 runs right to left) and the
+    site shapes."""
+    total = 0.0
+    for (half, i, kind), k in counts.items():
+        ml, d, mr = shapes[i]
+        dl, dr = mpo_shapes[i][0], mpo_shapes[i][3]
+        f_hop1 = 8 * (ml * ml * dl * d * mr + ml * d * dr * mr * mr) + 4 * ml * mr * dl * dr * d * d
+        if kind == 1:
+            total += k * f_hop1
+            continue
+        if half == 0:   # RQ of (ml) x (d mr); bond (ml x ml) between L[i-1] and the new R[i]
+            mb, db, qr = ml, dl, qr_flop(d * mr, ml)
+        else:           # QR of (ml d) x (mr); bond (mr x mr)
+            mb, db, qr = mr, dr, qr_flop(ml * d, mr)
+        total += k * 8 * db * mb * mb * 2 * mb + qr + f_hop1 + 8 * mb * mb * mb * d
+    return total
 
 
 def copy_calibration(mib=512, reps=3):
@@ -300,6 +372,7 @@ def main():
                           "n_gpus": a.gpus}))
         sys.exit(2)
 
+    import numpy as np
     import torch
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
@@ -363,8 +436,11 @@ def main():
     if not a.no_cpu:
         ref = res.get("fp64", r)
         log("cpu baseline sample")
-        out["cpu_baseline"] = cpu_baseline(ref["final"], ref["environ"], mpo, a.dt, a.cpu_reps,
-                                           rl["gemm_flop_per_step"])
+        fin = ref["final"]
+        counts = fin._env_cache[3][1]
+        fstep = step_flop(counts, [tuple(m.shape) for m in fin._mp], [np.asarray(w).shape for w in mpo])
+        out["algorithmic_flop_per_step"] = fstep
+        out["cpu_baseline"] = cpu_baseline(fin, ref["environ"], mpo, a.dt, a.cpu_reps, fstep)
     print(json.dumps(out), flush=True)
 
 

@@ -371,8 +371,12 @@ class Mps(MatrixProduct):
         prec = _PREC[self.evolve_config.precision]
         n = len(self)
         cache = self._env_cache
-        reuse = cache is not None and cache[0] is mpo and cache[1] == prec and not self.to_right \
-            and self.qnidx == n - 1
+        # a step starts at one end of the chain and returns there: right to
+        # left from the last site (left environments kept) or left to right
+        # from the first (right environments kept)
+        at_end = self.qnidx == (0 if self.to_right else n - 1)
+        reuse = cache is not None and cache[0] is mpo and cache[1] == prec and at_end \
+            and cache[4] == self.to_right
         # the sweep rewrites the cached environments in place: take them away
         # from self so that a second evolution of self rebuilds its own
         self._env_cache = None
@@ -399,7 +403,8 @@ class Mps(MatrixProduct):
         mps.evolve_config.local_steps = local_steps
         mps.evolve_config.fp32_matvecs = _fp32_matvecs(counts, switches if adaptive and not synchronous
                                                        else None)
-        mps._env_cache = (mpo, prec, environ, (complex(evolve_dt), dict(counts), synchronous, switch))
+        mps._env_cache = (mpo, prec, environ, (complex(evolve_dt), dict(counts), synchronous, switch),
+                          mps.to_right)
         return mps
 
     @adaptive_tdvp

@@ -4,8 +4,9 @@ spin up x vacuum expanded along H with coefficient 1e-16 (reference
 sbm/sbm.py:55-59), built exactly as bench.py builds it.
 
 * M = 128: two FP64 device steps against the numpy oracle on identical
-  tensors (reference mps/mps.py:1267-1404): <sigma_z> to 1e-8, every site
-  tensor to 1e-8 relative, Lanczos matvec counts, quantum numbers.
+  tensors (reference mps/mps.py:1267-1404): <sigma_z> to 1e-8, every
+  one-site reduced density matrix to 1e-10, fidelity 1 - 1e-12, Lanczos
+  matvec counts, quantum numbers.
 * M = 256: ten device steps of each mode.  FP64: |norm - 1| <= 1e-12 and
   the energy <H> constant to the Krylov tolerance.  Mixed (FP32 matvecs in
   the tail of every Lanczos iteration): <sigma_z> within 1e-8 of the FP64
@@ -54,8 +55,17 @@ def test_c4_m128_fp64_oracle_parity():
         print(f"[C4 M=128 step {k + 1}] sz dev {_sz_dev(dev):.12f} oracle {_sz_orc(st):.12f}", flush=True)
         assert abs(_sz_dev(dev) - _sz_orc(st)) < 1e-8
     _assert_steps_match(sd, so)
-    for a, b in zip(dev.numpy_sites(), st.sites):
-        assert np.linalg.norm(a - b) <= 1e-8 * np.linalg.norm(b)
+    # gauge-invariant state comparison: the site tensors themselves are not
+    # unique (directions of the bond spaces whose weight is at rounding level,
+    # left by the 1e-16 expansion, are fixed by rounding in the QR), so
+    # compare every one-site reduced density matrix and the overlap
+    rdm_dev = dev.calc_1site_rdm()
+    worst = max(np.abs(rdm_dev[i] - orc.calc_1site_rdm(st, i)).max() for i in range(len(st)))
+    ov = orc.dot([s.conj() for s in dev.numpy_sites()], st.sites)
+    fid = abs(ov) ** 2 / (orc.mp_norm(st) ** 2 * dev.mp_norm ** 2)
+    print(f"[C4 M=128] max |rdm_dev - rdm_oracle| {worst:.2e}, 1 - fidelity {1 - fid:.2e}", flush=True)
+    assert worst <= 1e-10
+    assert 1 - fid <= 1e-12
     assert [np.array(q).tolist() for q in dev.qn] == [np.array(q).tolist() for q in st.qn]
     assert dev.qnidx == st.qnidx and dev.to_right == st.to_right
 

@@ -227,7 +227,7 @@ def test_async_budget_redo_matches_sync():
     sync = sync.evolve(mpo, 0.5)
     # shrink the recorded budgets so every asynchronous call runs out
     cache = first._env_cache
-    first._env_cache = cache[:3] + ((cache[3][0], {k: 1 for k in cache[3][1]}, False, None),)
+    first._env_cache = cache[:3] + ((cache[3][0], {k: 1 for k in cache[3][1]}, False, None),) + cache[4:]
     before = Mps.krylov_redo_count
     redone = first.evolve(mpo, 0.5)
     assert Mps.krylov_redo_count == before + 1
