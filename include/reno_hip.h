@@ -81,6 +81,19 @@ size_t reno_qr_workspace_bytes(int rows, int cols, int nblocks, const reno_qn_bl
 int reno_hop_apply(const reno_hop_desc* op, const void* c_in, void* c_out,
                    void* ws, size_t ws_bytes, void* stream);
 
+/* ---- hop_apply without allocation --------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  Computes
+ * exactly what reno_hop_apply computes, with the same kernels, but its
+ * scratch lives in a static device array of 96 MiB (one per device, held
+ * until the process ends) instead of a caller workspace; it calls no
+ * allocation routine.  Operators whose reno_hop_workspace_bytes exceeds
+ * 96 MiB are rejected with RENO_EINVAL (the FP64 one-site matvec at
+ * M = 256, d = 10, D = 3 needs 90 MiB).  All calls on one device share the
+ * array: calls that overlap on a device race and must be serialised. */
+int reno_hop_apply_noalloc(const reno_hop_desc* op, const void* c_in, void* c_out,
+                           void* stream);
+
 /* ---- contract_one_site (replaces lib.py:169-250) --------------------
  * domain 0 = "L": out[f,g,h] = sum env[a,b,c] conj(ms[a,d,f]) W[b,d,e,g] ms[c,e,h]
  *   env_in (Ml, Dl, Ml) -> env_out (Mr, Dr, Mr)

@@ -563,6 +563,10 @@ static int harvest_markers() {
   return RENO_OK;
 }
 
+static thread_local int t_hook_suspended = 0;
+TimingHookSuspend::TimingHookSuspend() { ++t_hook_suspended; }
+TimingHookSuspend::~TimingHookSuspend() { --t_hook_suspended; }
+
 int launch_gemm(const GemmCall& g, hipStream_t stream) {
   if (g.M < 0 || g.N < 0 || g.K < 0 || g.batch < 0) {
     set_error("gemm: negative size M=%d N=%d K=%d batch=%d", g.M, g.N, g.K, g.batch);
@@ -643,7 +647,7 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
   bool prof = false, timed = false;
   {
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    if (g_prof_period > 0 && g_exec) {
+    if (g_prof_period > 0 && g_exec && !t_hook_suspended) {
       // every slot in use: collect the markers before reusing them
       if (g_prof.size() - g_prof_harvested == kExecSlots) {
         const int st = harvest_markers();

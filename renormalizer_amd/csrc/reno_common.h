@@ -94,6 +94,13 @@ struct GemmCall {
 };
 
 int launch_gemm(const GemmCall& g, hipStream_t stream);
+// While an object of this type lives on a thread, the GEMM launches of that
+// thread bypass the timing hook (which allocates its marker array and
+// synchronises the device when it harvests markers).
+struct TimingHookSuspend {
+  TimingHookSuspend();
+  ~TimingHookSuspend();
+};
 // bytes of split-K scratch launch_gemm would use for this shape (0: no split)
 size_t splitk_bytes(int prec, int M, int N, int K, int batch);
 

@@ -32,7 +32,7 @@ EXPORTS = (
     "reno_version", "reno_status_string", "reno_last_error",
     "reno_hop_workspace_bytes", "reno_krylov_workspace_bytes",
     "reno_env_workspace_bytes", "reno_qr_workspace_bytes",
-    "reno_hop_apply", "reno_env_update", "reno_env_close", "reno_expm_krylov",
+    "reno_hop_apply", "reno_hop_apply_noalloc", "reno_env_update", "reno_env_close", "reno_expm_krylov",
     "reno_expm_krylov_async", "reno_expm_krylov_mixed_async", "reno_block_qr", "reno_heff_diag", "reno_heff_dense", "reno_zgemm",
     "reno_probe_mfma16", "reno_prof_enable", "reno_prof_collect", "reno_prof_kind_info",
     "reno_prof_cancelled",
@@ -101,6 +101,8 @@ def load_library(path: str = LIB_PATH):
         lib.reno_qr_workspace_bytes.argtypes = [c_int, c_int, c_int, ctypes.POINTER(QnBlock)]
         lib.reno_hop_apply.restype = c_int
         lib.reno_hop_apply.argtypes = [ctypes.POINTER(HopDesc), vp, vp, vp, c_size, vp]
+        lib.reno_hop_apply_noalloc.restype = c_int
+        lib.reno_hop_apply_noalloc.argtypes = [ctypes.POINTER(HopDesc), vp, vp, vp]
         lib.reno_env_update.restype = c_int
         lib.reno_env_update.argtypes = [c_int] * 7 + [vp, vp, vp, vp, vp, vp, c_size, vp]
         lib.reno_expm_krylov.restype = c_int
