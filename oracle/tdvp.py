@@ -431,6 +431,36 @@ def m_trunc(sigma, bond, max_dim=None, threshold=None):
     return min(cands)
 
 
+def truncate_two_site(ms2, qnbigl, qnbigr, qntot, to_right, max_dim=None, threshold=None):
+    """The truncation of the two-site TDVP-PS step (reference mp.py:651-780
+    with one root and percent 0; mps.py:1406-1517): blocked SVD of the
+    (M_l d0) x (d1 M_r) matrix (svd_qn), the number kept by compute_m_trunc
+    (m_trunc: fixed, threshold or both), basis selection of the largest
+    weights (select_basis).  Moving right the left factor keeps the singular
+    vectors and the right one absorbs the weights, moving left the reverse.
+    Returns (left site (M_l, d0, m), right site (m, d1, M_r), m, quantum
+    numbers of the new bond).  Pinned by: the product left . right is the
+    best rank-m approximation of the dense two-site matrix (Eckart-Young:
+    the discarded weight equals the norm of its smallest singular values),
+    the kept count of the threshold branch, exact zeros outside the quantum
+    number blocks."""
+    system = "L" if to_right else "R"
+    u, su, qnlnew, v, sv, qnrnew = svd_qn(ms2, qnbigl, qnbigr, qntot, system=system)
+    lshape = list(qnbigl.shape[:-1])
+    rshape = list(qnbigr.shape[:-1])
+    if to_right:
+        mt = m_trunc(su, None, max_dim, threshold)
+        ms, msdim, msqn, comp = select_basis(u, su, qnlnew, v, mt)
+        left = ms.reshape(lshape + [msdim])
+        right = np.moveaxis(comp.reshape(rshape + [msdim]), -1, 0)
+    else:
+        mt = m_trunc(sv, None, max_dim, threshold)
+        ms, msdim, msqn, comp = select_basis(v, sv, qnrnew, u, mt)
+        right = np.moveaxis(ms.reshape(rshape + [msdim]), -1, 0)
+        left = comp.reshape(lshape + [msdim])
+    return left, right, msdim, msqn
+
+
 def tdvp_ps2(state: State, mpo, evolve_dt, max_dim=None, threshold=None):
     """One two-site TDVP-PS time step, reference mps/mps.py:1406-1517 with
     ivp_solver="krylov" and real ``evolve_dt``: per bond the two-site
@@ -439,8 +469,9 @@ def tdvp_ps2(state: State, mpo, evolve_dt, max_dim=None, threshold=None):
     the site left behind followed by a QR moving the centre on (mp.py:
     890-908); then normalize("mps_only").  Returns (new_state,
     local_steps).  Pinned by: exactness vs the dense propagator when no
-    truncation happens, the 1-site oracle's closed forms, and the
-    truncation against a direct SVD of the dense two-site tensor."""
+    truncation happens, the 1-site oracle's closed forms, and its
+    truncation (truncate_two_site) against a direct SVD of the dense
+    two-site tensor."""
     mps = state.copy()
     n = len(mps)
     environ = Environ(mps.sites, mpo)
@@ -467,21 +498,11 @@ def tdvp_ps2(state: State, mpo, evolve_dt, max_dim=None, threshold=None):
             qnr = np.array(mps.qn[c1 + 1])
             qnbigl = add_outer(qnl, mps.sigmaqn[c0])
             qnbigr = add_outer(mps.sigmaqn[c1], qnr)
-            system = "L" if mps.to_right else "R"
-            u, su, qnlnew, v, sv, qnrnew = svd_qn(mps_t.reshape(shape2), qnbigl, qnbigr, mps.qntot,
-                                                  system=system)
-            if mps.to_right:
-                mt = m_trunc(su, c0 + 1, max_dim, threshold)
-                ms, msdim, msqn, comp = select_basis(u, su, qnlnew, v, mt)
-                mps.sites[c0] = ms.reshape(list(qnbigl.shape[:-1]) + [msdim])
-                mps.sites[c1] = np.moveaxis(comp.reshape(list(qnbigr.shape[:-1]) + [msdim]), -1, 0)
-                mps.qnidx = c1
-            else:
-                mt = m_trunc(sv, c1, max_dim, threshold)
-                ms, msdim, msqn, comp = select_basis(v, sv, qnrnew, u, mt)
-                mps.sites[c1] = np.moveaxis(ms.reshape(list(qnbigr.shape[:-1]) + [msdim]), -1, 0)
-                mps.sites[c0] = comp.reshape(list(qnbigl.shape[:-1]) + [msdim])
-                mps.qnidx = c0
+            left, right, msdim, msqn = truncate_two_site(mps_t.reshape(shape2), qnbigl, qnbigr,
+                                                         mps.qntot, mps.to_right, max_dim, threshold)
+            mps.sites[c0] = left
+            mps.sites[c1] = right
+            mps.qnidx = c1 if mps.to_right else c0
             mps.qn[c1] = np.array(msqn).reshape(msdim, -1)
             if imps == last:
                 continue
@@ -498,7 +519,7 @@ def tdvp_ps2(state: State, mpo, evolve_dt, max_dim=None, threshold=None):
             # _push_cano(c2): QR moving the centre to the next site
             qnbigl1, qnbigr1 = mps.get_big_qn(c2)
             uq, qnlset, vq, qnrset = svd_qn(mps.sites[c2], qnbigl1, qnbigr1, mps.qntot, QR=True,
-                                            system=system, full_matrices=False)
+                                            system="L" if mps.to_right else "R", full_matrices=False)
             if mps.to_right:
                 mps.sites[c2 + 1] = np.tensordot(vq.T, mps.sites[c2 + 1], axes=1)
                 mps.sites[c2] = uq.reshape(list(shape1[:-1]) + [uq.shape[1]])

@@ -1,105 +1,113 @@
-"""Symbolic operators (reference model/op.py:11-528).
+"""Symbolic operators for sum-of-products Hamiltonians (the interface of
+reference model/op.py:11-528).
 
-``Op`` is an immutable product of elementary symbols with DoF names, a
-factor and per-symbol quantum numbers; ``OpSum`` is a list of ``Op``.  The
-behaviour the MPO construction relies on (splitting into per-site
-elementary operators, default quantum numbers of ``a^\\dagger``/``a``,
-equality/hash through ``to_tuple``) follows the reference.
+An ``Op`` is a product of elementary symbols ("a^\\dagger", "x", "sigma_z",
+...), each attached to one degree of freedom (DoF) and carrying a quantum
+number, times a scalar factor.  An ``OpSum`` is a list of ``Op`` terms.  The
+MPO construction only needs: the per-site split of a product
+(``split_elementary``), identity squeezing, merging of equal terms
+(``simplify``), the default quantum numbers of the electron ladder
+operators, and equality / hashing by value.  The printed form ``Op('X Y',
+[0, 1], 0.1)`` is the reference's.
 """
-from collections import defaultdict
-from itertools import chain
-from typing import List, Tuple, Union
+from itertools import groupby
+from typing import Dict, List, Tuple, Union
 
 import numpy as np
 
 from renormalizer_amd.utils import Quantity
 
+# quantum numbers the electron creation / annihilation symbols carry when
+# none are given; every other symbol conserves them
+_LADDER_QN = {r"a^\dagger": 1, "a": -1}
+# the displacement written with spaces is one symbol
+_JOINED = (r"b^\dagger + b", r"b^\dagger+b")
 
-def _default_qn(symbol: str) -> int:
-    if symbol == r"a^\dagger":
-        return 1
-    if symbol == "a":
-        return -1
-    return 0
+
+def _tokens(symbol: str) -> List[str]:
+    return symbol.replace(*_JOINED).split(" ")
+
+
+def _as_number(x):
+    """``x`` as a Python scalar, or None when it is not a plain number."""
+    if isinstance(x, np.generic):
+        x = x.item()
+    if isinstance(x, (int, float, complex)) and not isinstance(x, bool):
+        return x
+    return None
+
+
+def _is_zero(x) -> bool:
+    if isinstance(x, np.ndarray):
+        return x.shape == () and x.item() == 0
+    n = _as_number(x)
+    return n is not None and n == 0
 
 
 class Op:
-    @classmethod
-    def product(cls, op_list: List["Op"]) -> "Op":
-        symbol = " ".join(op.symbol for op in op_list)
-        dofs = list(chain.from_iterable(op.dofs for op in op_list))
-        factor = np.prod([op.factor for op in op_list])
-        qn = list(chain.from_iterable(op.qn_list for op in op_list))
-        return cls(symbol, dofs, factor, qn)
+    """``factor * s_0(dof_0) s_1(dof_1) ...``.
 
-    @classmethod
-    def identity(cls, dof, qn_size=1, factor=1.0):
-        if isinstance(dof, list):
-            return cls(" ".join(["I"] * len(dof)), dof, factor=factor,
-                       qn=[np.zeros(qn_size, dtype=int)] * len(dof))
-        return cls("I", dof, factor=factor, qn=[np.zeros(qn_size, dtype=int)])
+    ``symbol`` is the space-separated product, ``dof`` one DoF name (shared by
+    every symbol) or a list with one name per symbol, ``qn`` one quantum
+    number per symbol (a list; a scalar for a single symbol), defaulting to
+    +1 for "a^\\dagger", -1 for "a", 0 otherwise."""
 
     def __init__(self, symbol: str, dof, factor: Union[float, complex, Quantity] = 1.0,
                  qn: Union[List, int] = None):
         if not isinstance(symbol, str):
             raise TypeError(f"symbol should be a str. Got {symbol} as {type(symbol)}")
-        self.symbol = symbol
-        # "b^\dagger + b" is one simple symbol; everything else splits on spaces
-        self.split_symbol: List[str] = symbol.replace(r"b^\dagger + b", r"b^\dagger+b").split(" ")
-        nsym = len(self.split_symbol)
-        if nsym == 1:
-            if isinstance(dof, list):
-                assert len(dof) == 1
-                dofs = dof
-            else:
-                dofs = [dof]
-            if isinstance(qn, list):
-                if len(qn) != 1:
-                    raise ValueError(f"Incompatible sizes of quantum number {qn} and symbol {self.split_symbol}")
-                qn_list = qn
-            elif qn is None:
-                qn_list = None
-            else:
-                qn_list = [qn]
+        tokens = _tokens(symbol)
+        n = len(tokens)
+        if isinstance(dof, list):
+            if n == 1 and len(dof) != 1:
+                raise AssertionError(f"one symbol {symbol!r} with {len(dof)} DoF names")
+            if len(dof) != n:
+                raise ValueError(f"symbol and DoF name not match: {tokens} vs {dof}")
+            dofs = list(dof)
         else:
-            if isinstance(dof, list):
-                if nsym != len(dof):
-                    raise ValueError("symbol and DoF name not match")
-                dofs = dof
-            else:
-                dofs = [dof] * nsym
-            if isinstance(qn, list):
-                if nsym != len(qn):
-                    raise ValueError("symbol and qn length not match")
-                qn_list = qn
-            elif qn is None:
-                qn_list = None
-            else:
-                raise ValueError("qn should be a list.")
-        if qn_list is None:
-            qn_list = [_default_qn(s) for s in self.split_symbol]
+            dofs = [dof] * n
         for d in dofs:
             if d.__hash__ is None:
                 raise ValueError(f"dof name should be hashable. Got {d}.")
-        self.dofs: List = dofs
+        if qn is None:
+            qns = [_LADDER_QN.get(t, 0) for t in tokens]
+        elif isinstance(qn, list):
+            if len(qn) != n:
+                raise ValueError(f"Incompatible sizes of quantum number {qn} and symbol {tokens}")
+            qns = qn
+        elif n == 1:
+            qns = [qn]
+        else:
+            raise ValueError(f"qn of a product of {n} symbols should be a list, got {qn!r}")
         if isinstance(factor, Quantity):
             factor = factor.as_au()
+        self.symbol = symbol
+        self.split_symbol: List[str] = tokens
+        self.dofs: List = dofs
         self._factor = factor + 0.0
-        self.qn_list: List[np.ndarray] = [np.array(q).reshape(-1) for q in qn_list]
+        self.qn_list: List[np.ndarray] = [np.array(q).reshape(-1) for q in qns]
 
-    def split_elementary(self, dof_to_siteidx) -> Tuple[List["Op"], Union[float, complex]]:
-        """Group the symbols by MPS site (order kept within a site, sites in
-        increasing order), factors set to 1 (reference op.py:185-228)."""
-        if len(self.dofs) == 1:
-            return [Op(self.symbol, self.dofs, qn=self.qn_list)], self.factor
-        grouped = defaultdict(list)
-        for sym, name, qn in zip(self.split_symbol, self.dofs, self.qn_list):
-            site = dof_to_siteidx.get(name)
-            if site is None:
-                raise ValueError(f"Unknown DoF name {name} in {self}.")
-            grouped[site].append(Op(sym, name, qn=qn))
-        return [Op.product(grouped[s]) for s in sorted(grouped)], self.factor
+    # ---- constructors ---------------------------------------------------
+    @classmethod
+    def product(cls, op_list: List["Op"]) -> "Op":
+        """One operator: the symbols of ``op_list`` in order, factors multiplied."""
+        factor = 1.0
+        for op in op_list:
+            factor = factor * op.factor
+        return cls(" ".join(op.symbol for op in op_list),
+                   [d for op in op_list for d in op.dofs], factor,
+                   [q for op in op_list for q in op.qn_list])
 
+    @classmethod
+    def identity(cls, dof, qn_size=1, factor=1.0):
+        names = dof if isinstance(dof, list) else [dof]
+        return cls(" ".join("I" for _ in names), names, factor=factor,
+                   qn=[np.zeros(qn_size, dtype=int) for _ in names])
+
+    def _with_factor(self, factor) -> "Op":
+        return Op(self.symbol, self.dofs, factor, self.qn_list)
+
+    # ---- properties -----------------------------------------------------
     @property
     def factor(self):
         return self._factor
@@ -114,148 +122,161 @@ class Op:
 
     @property
     def is_identity(self) -> bool:
-        return set(self.split_symbol) == {"I"}
+        return all(t == "I" for t in self.split_symbol)
 
-    def squeeze_identity(self):
+    def squeeze_identity(self) -> "Op":
+        """The same operator without identity symbols (one identity kept if
+        nothing else is left)."""
         if self.is_identity:
-            return self.__class__.identity(self.dofs[0], factor=self.factor, qn_size=self.qn_size)
-        syms, dofs, qns = [], [], []
-        for s, d, q in zip(self.split_symbol, self.dofs, self.qn_list):
-            if s == "I":
-                continue
-            syms.append(s)
-            dofs.append(d)
-            qns.append(q)
-        return Op(" ".join(syms), dofs, self.factor, qns)
+            return Op.identity(self.dofs[0], qn_size=self.qn_size, factor=self.factor)
+        kept = [(t, d, q) for t, d, q in zip(self.split_symbol, self.dofs, self.qn_list) if t != "I"]
+        return Op(" ".join(t for t, _, _ in kept), [d for _, d, _ in kept], self.factor,
+                  [q for _, _, q in kept])
 
-    def same_term(self, other) -> bool:
+    def split_elementary(self, dof_to_siteidx: Dict) -> Tuple[List["Op"], Union[float, complex]]:
+        """Per-site operators of the product with unit factors, sites in
+        increasing order and symbols in their original order within a site,
+        and the factor (the contract of reference op.py:185-228)."""
+        if len(self.dofs) == 1:
+            return [Op(self.symbol, self.dofs, qn=self.qn_list)], self.factor
+        placed = []
+        for pos, (t, d, q) in enumerate(zip(self.split_symbol, self.dofs, self.qn_list)):
+            if d not in dof_to_siteidx:
+                raise ValueError(f"Unknown DoF name {d} in {self}.")
+            placed.append((dof_to_siteidx[d], pos, Op(t, d, qn=q)))
+        placed.sort(key=lambda x: (x[0], x[1]))
+        per_site = [Op.product([op for _, _, op in grp]) for _, grp in groupby(placed, key=lambda x: x[0])]
+        return per_site, self.factor
+
+    # ---- identity of terms ----------------------------------------------
+    def same_term(self, other: "Op") -> bool:
         return self.symbol == other.symbol and self.dofs == other.dofs
 
     def to_tuple(self):
-        return self.symbol, tuple(self.dofs), self.factor, tuple(tuple(t) for t in self.qn_list)
+        return self.symbol, tuple(self.dofs), self.factor, tuple(tuple(q) for q in self.qn_list)
 
     def __hash__(self):
         return hash(self.to_tuple())
 
     def __eq__(self, other):
-        return self.to_tuple() == other.to_tuple()
+        return isinstance(other, Op) and self.to_tuple() == other.to_tuple()
 
     def __str__(self):
-        ret = ", ".join([repr(self.symbol), str(self.dofs), str(self.factor)])
-        if not np.all(np.array(self.qn_list) == 0):
-            ret += f", {[q.tolist() for q in self.qn_list]}"
-        return f"Op({ret})"
+        parts = [repr(self.symbol), str(self.dofs), str(self.factor)]
+        if any(np.any(q != 0) for q in self.qn_list):
+            parts.append(str([q.tolist() for q in self.qn_list]))
+        return "Op(" + ", ".join(parts) + ")"
 
     __repr__ = __str__
 
+    # ---- arithmetic -----------------------------------------------------
     def __add__(self, other):
-        if isinstance(other, (int, float)) and other == 0:
-            return OpSum([self])
-        if isinstance(other, np.ndarray) and np.array_equal(other, np.array(0)):
+        if _is_zero(other):
             return OpSum([self])
         if isinstance(other, Op):
             return OpSum([self, other])
         if isinstance(other, list):
-            return OpSum([self] + other)
-        raise TypeError(f"Unknown operand type {type(other)}")
+            return OpSum([self, *other])
+        raise TypeError(f"cannot add {type(other)} to an Op")
 
     def __radd__(self, other):
-        if (isinstance(other, (int, float)) and other == 0) or \
-                (isinstance(other, np.ndarray) and np.array_equal(other, np.array(0))):
+        if _is_zero(other):
             return OpSum([self])
-        raise TypeError(f"Unknown operand type {type(other)}")
+        raise TypeError(f"cannot add an Op to {type(other)}")
 
     def __neg__(self):
-        return Op(self.symbol, self.dofs, -self.factor, self.qn_list)
+        return self._with_factor(-self.factor)
 
     def __sub__(self, other):
         return self + (-other)
 
     def __mul__(self, other):
-        if isinstance(other, np.generic):
-            other = other.item()
         if isinstance(other, Op):
             return Op.product([self, other])
-        if isinstance(other, (int, float, complex)):
-            return Op(self.symbol, self.dofs, self.factor * other, self.qn_list)
+        number = _as_number(other)
+        if number is not None:
+            return self._with_factor(self.factor * number)
         if isinstance(other, list):
-            for item in other:
-                if not isinstance(item, Op):
-                    raise TypeError(f"Operand must be a list of `Op`. Got {type(item)}")
-            return OpSum([self * item for item in other])
-        raise TypeError(f"Unsupported type: {type(other)}")
+            if not all(isinstance(o, Op) for o in other):
+                raise TypeError("an Op multiplies a list of Op only")
+            return OpSum([self * o for o in other])
+        raise TypeError(f"cannot multiply an Op by {type(other)}")
 
     def __rmul__(self, other):
-        if isinstance(other, (int, float, complex, np.generic)):
-            return self * other
+        number = _as_number(other)
+        if number is not None:
+            return self._with_factor(number * self.factor)
         if isinstance(other, list):
             return OpSum(other) * self
-        raise TypeError(f"Unknwon type {type(other)}")
+        raise TypeError(f"cannot multiply {type(other)} by an Op")
 
 
 class OpSum(list):
+    """A sum of ``Op`` terms (a list); arithmetic distributes over terms."""
+
     @classmethod
     def product(cls, op_list):
-        if len(op_list) == 0:
+        if not op_list:
             return cls()
-        prod = op_list[0]
+        result = op_list[0]
         for op in op_list[1:]:
-            prod = prod * op
-        return prod
+            result = result * op
+        return result
 
     def copy(self):
-        return OpSum(super().copy())
+        return OpSum(self)
 
     def simplify(self, atol=0) -> "OpSum":
-        """Combine identical terms, then drop |factor| <= atol (reference
-        op.py:451-490, same term order)."""
-        pending = OpSum(reversed([op.squeeze_identity() for op in self]))
-        merged = []
-        while pending:
-            op = pending.pop()
-            same = [i for i, o in enumerate(pending) if op.same_term(o)]
-            op = Op(op.symbol, op.dofs, op.factor + sum(pending[i].factor for i in same), op.qn_list)
-            for k, i in enumerate(same):
-                pending.pop(i - k)
-            merged.append(op)
-        return OpSum([op for op in merged if np.abs(op.factor) > atol])
+        """Identity-squeezed terms with equal symbol and DoFs merged into the
+        first of them (factors added in term order), in order of first
+        appearance; terms with |factor| <= atol dropped."""
+        first: Dict = {}
+        for op in self:
+            op = op.squeeze_identity()
+            key = (op.symbol, tuple(op.dofs))
+            if key in first:
+                head, total = first[key]
+                first[key] = (head, total + op.factor)
+            else:
+                first[key] = (op, op.factor)
+        return OpSum(head._with_factor(total) for head, total in first.values() if np.abs(total) > atol)
 
     def __add__(self, other):
-        if not isinstance(other, (Op, list)):
-            raise TypeError("OpSum can only add with `Op` or list of `Op`")
         if isinstance(other, Op):
-            other = [other]
-        return OpSum(super().__add__(other))
+            return OpSum([*self, other])
+        if isinstance(other, list):
+            return OpSum([*self, *other])
+        raise TypeError("an OpSum adds an Op or a list of Op only")
 
     def __iadd__(self, other):
         if isinstance(other, Op):
             self.append(other)
-            return self
-        return super().__iadd__(other)
+        else:
+            self.extend(other)
+        return self
 
     def __neg__(self):
-        return OpSum([-op for op in self])
+        return OpSum(-op for op in self)
 
     def __sub__(self, other):
         return self + (-other)
 
     def __mul__(self, other):
         if isinstance(other, list):
-            res = []
-            for op1 in self:
-                res.extend(op1 * other)
-            return OpSum(res)
-        if isinstance(other, (int, float, complex, np.generic, Op)):
-            return OpSum([op * other for op in self])
-        return OpSum(super().__mul__(other))
+            return OpSum(p for left in self for p in left * other)
+        if isinstance(other, Op) or _as_number(other) is not None:
+            return OpSum(op * other for op in self)
+        return OpSum(list.__mul__(self, other))
 
     def __rmul__(self, other):
-        if isinstance(other, (int, float, complex, np.generic)):
-            return self * other
-        return OpSum(super().__rmul__(other))
+        if _as_number(other) is not None:
+            return OpSum(other * op for op in self)
+        return OpSum(list.__rmul__(self, other))
 
     def __truediv__(self, other):
-        assert isinstance(other, (int, float, complex, np.generic))
-        return self * (1 / other)
+        number = _as_number(other)
+        if number is None:
+            raise TypeError(f"an OpSum divides by a number only, not {type(other)}")
+        return self * (1 / number)
 
     __array_ufunc__ = None

@@ -332,3 +332,73 @@ def test_tdvp_ps2_holstein_qn_exact():
     for k in range(1, 6):
         st, _ = orc.tdvp_ps2(st, w, 0.1, max_dim=1000)
         np.testing.assert_allclose(_to_dense(st), exact[k], atol=1e-8)
+
+
+def _two_site_case(rng, qn):
+    """A random two-site tensor (M_l, d0, d1, M_r) with M_l = 5, d = 3, M_r = 4;
+    with qn, one-particle quantum numbers (total 1) and the forbidden
+    entries zeroed, else all quantum numbers 0."""
+    ml, d0, d1, mr = 5, 3, 3, 4
+    t = rng.normal(size=(ml, d0, d1, mr)) + 1j * rng.normal(size=(ml, d0, d1, mr))
+    if qn:
+        ql = rng.integers(0, 2, ml).reshape(-1, 1)
+        s0 = np.array([[0], [1], [0]])
+        s1 = np.array([[0], [0], [1]])
+        qr = rng.integers(0, 2, mr).reshape(-1, 1)   # right-block numbers (qntot - left)
+        qnbigl = orc.add_outer(ql, s0)
+        qnbigr = orc.add_outer(s1, qr)
+        tot = np.array([1])
+        allowed = (qnbigl[:, :, 0][:, :, None, None] + qnbigr[:, :, 0][None, None, :, :]) == 1
+        t = t * allowed
+    else:
+        qnbigl = np.zeros((ml, d0, 1), dtype=int)
+        qnbigr = np.zeros((d1, mr, 1), dtype=int)
+        tot = np.array([0])
+    return t, qnbigl, qnbigr, tot
+
+
+@pytest.mark.parametrize("qn", [False, True])
+@pytest.mark.parametrize("to_right", [True, False])
+def test_two_site_truncation_is_best_low_rank(qn, to_right):
+    """truncate_two_site against a direct SVD of the dense two-site matrix:
+    the kept product is the best rank-m approximation (the discarded weight
+    equals the norm of the smallest singular values), for several m and both
+    sweep directions, with and without quantum-number blocks."""
+    rng = np.random.default_rng(11 + 2 * qn + to_right)
+    t, qnbigl, qnbigr, tot = _two_site_case(rng, qn)
+    ml, d0, d1, mr = t.shape
+    a = t.reshape(ml * d0, d1 * mr)
+    s = np.linalg.svd(a, compute_uv=False)
+    for m in (1, 3, 6):
+        left, right, msdim, msqn = orc.truncate_two_site(t, qnbigl, qnbigr, tot, to_right, max_dim=m)
+        assert msdim == min(m, np.count_nonzero(s > 1e-12) if qn else min(a.shape))
+        approx = np.tensordot(left, right, axes=1).reshape(a.shape)
+        np.testing.assert_allclose(np.linalg.norm(a - approx), np.sqrt(np.sum(s[msdim:] ** 2)),
+                                   rtol=1e-10, atol=1e-12)
+        # the orthonormal factor is an isometry on the kept bond
+        iso = left.reshape(-1, msdim) if to_right else right.reshape(msdim, -1).T
+        np.testing.assert_allclose(iso.conj().T @ iso, np.eye(msdim), atol=1e-12)
+        if qn:   # exact zeros outside the allowed blocks of the new bond
+            q = np.array(msqn)[:, 0]
+            if not to_right:   # moving left the bond carries right-block numbers
+                q = tot[0] - q
+            okl = qnbigl[:, :, 0][:, :, None] == q[None, None, :]
+            assert np.all(left[~okl] == 0)
+
+
+def test_two_site_truncation_threshold_branch():
+    """m_trunc's threshold branch (reference configs.py:195-199): keep the
+    singular values whose share of the norm exceeds the threshold, and the
+    smaller of the two counts with both criteria."""
+    rng = np.random.default_rng(5)
+    t, qnbigl, qnbigr, tot = _two_site_case(rng, False)
+    a = t.reshape(t.shape[0] * t.shape[1], -1)
+    s = np.linalg.svd(a, compute_uv=False)
+    normed = s / np.linalg.norm(s)
+    for thr in (0.05, 0.2, 0.4):
+        expect = int(np.sum(normed > thr))
+        assert orc.m_trunc(s, 0, threshold=thr) == expect
+        _, _, m, _ = orc.truncate_two_site(t, qnbigl, qnbigr, tot, True, threshold=thr)
+        assert m == expect
+        _, _, m2, _ = orc.truncate_two_site(t, qnbigl, qnbigr, tot, True, max_dim=2, threshold=thr)
+        assert m2 == min(2, expect)
