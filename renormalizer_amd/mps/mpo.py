@@ -7,6 +7,7 @@ matrix is complex), layout (D_l, d_out, d_in, D_r), kept on the host; the
 device copies used by the TDVP-PS kernels (float64 for the FP64 mode,
 float32 for the mixed mode) are made on first use and cached.
 """
+import threading
 from typing import List, Union
 
 import numpy as np
@@ -18,6 +19,11 @@ from renormalizer_amd.mps.svd_qn import add_outer
 from renormalizer_amd.mps.symbolic_mpo import (construct_symbolic_mpo, symbolic_mo_to_numeric_mo,
                                                terms_to_table)
 from renormalizer_amd.utils import Quantity
+
+
+# the device copies are made once per operator even when several host
+# threads (concurrent-stream batch evolution) ask for them at the same time
+_DEVICE_CACHE_LOCK = threading.Lock()
 
 
 class Mpo(MatrixProduct):
@@ -98,16 +104,17 @@ class Mpo(MatrixProduct):
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device())
         key = (prec, str(device))
-        cached = getattr(self, "_device_cache", None)
-        if cached is None:
-            self._device_cache = cached = {}
-        if key not in cached:
-            if np.iscomplexobj(np.asarray(self._mp[0])) or any(np.iscomplexobj(m) for m in self._mp):
-                raise NotImplementedError("complex MPO tensors are not supported by the device kernels")
-            dt = torch.float64 if prec == 0 else torch.float32
-            cached[key] = [torch.from_numpy(np.ascontiguousarray(m, dtype=np.float64)).to(device=device, dtype=dt)
-                           for m in self._mp]
-        return cached[key]
+        with _DEVICE_CACHE_LOCK:
+            cached = getattr(self, "_device_cache", None)
+            if cached is None:
+                self._device_cache = cached = {}
+            if key not in cached:
+                if np.iscomplexobj(np.asarray(self._mp[0])) or any(np.iscomplexobj(m) for m in self._mp):
+                    raise NotImplementedError("complex MPO tensors are not supported by the device kernels")
+                dt = torch.float64 if prec == 0 else torch.float32
+                cached[key] = [torch.from_numpy(np.ascontiguousarray(m, dtype=np.float64)).to(device=device, dtype=dt)
+                               for m in self._mp]
+            return cached[key]
 
     def apply(self, mp: MatrixProduct, canonicalise: bool = False) -> MatrixProduct:
         """MPO x MPS on host tensors (initial-state preparation; reference

@@ -18,6 +18,7 @@ left-to-right half-sweep produced (the final normalisation only rescales the
 last site), and the rebuilt right environments are overwritten before use.
 """
 import logging
+import threading
 from functools import wraps
 from typing import Dict, List, Union
 
@@ -757,26 +758,29 @@ class Mps(MatrixProduct):
 
 
 _ID_CACHE = {}
+_ID_LOCK = threading.Lock()
 
 
 def _identity_w(d, dev):
     key = ("id", d, str(dev))
-    if key not in _ID_CACHE:
-        _ID_CACHE[key] = torch.eye(d, dtype=torch.float64, device=dev).reshape(1, d, d, 1).contiguous()
-    return _ID_CACHE[key]
+    with _ID_LOCK:
+        if key not in _ID_CACHE:
+            _ID_CACHE[key] = torch.eye(d, dtype=torch.float64, device=dev).reshape(1, d, d, 1).contiguous()
+        return _ID_CACHE[key]
 
 
 def _pair_w(d, dev):
     """W[0, i', j', i*d + j] = delta(i', i) delta(j', j): the environment
     update with it keeps the (bra, ket) physical pair open."""
     key = ("pair", d, str(dev))
-    if key not in _ID_CACHE:
-        w = np.zeros((1, d, d, d * d))
-        for i in range(d):
-            for j in range(d):
-                w[0, i, j, i * d + j] = 1.0
-        _ID_CACHE[key] = torch.from_numpy(w).to(dev)
-    return _ID_CACHE[key]
+    with _ID_LOCK:
+        if key not in _ID_CACHE:
+            w = np.zeros((1, d, d, d * d))
+            for i in range(d):
+                for j in range(d):
+                    w[0, i, j, i * d + j] = 1.0
+            _ID_CACHE[key] = torch.from_numpy(w).to(dev)
+        return _ID_CACHE[key]
 
 
 def _identity_environments(ket, dev, upto=None):

@@ -10,6 +10,8 @@
   used only while preparing initial states (canonicalise / compress of the
   starting MPS) before the state is uploaded.
 """
+import threading
+
 import numpy as np
 import scipy.linalg
 
@@ -45,6 +47,7 @@ def qn_blocks(qnbigl, qnbigr, qntot):
 
 _PLAN_CACHE = {}
 _PLAN_CACHE_MAX = 256
+_PLAN_LOCK = threading.Lock()
 
 
 def svd_qn_plan(qnbigl, qnbigr, qntot):
@@ -54,7 +57,8 @@ def svd_qn_plan(qnbigl, qnbigr, qntot):
     qnbigl = np.ascontiguousarray(qnbigl)
     qnbigr = np.ascontiguousarray(qnbigr)
     key = (qnbigl.shape, qnbigl.tobytes(), qnbigr.shape, qnbigr.tobytes(), tuple(np.asarray(qntot).tolist()))
-    plan = _PLAN_CACHE.get(key)
+    with _PLAN_LOCK:
+        plan = _PLAN_CACHE.get(key)
     if plan is None:
         rows = int(np.prod(qnbigl.shape[:-1]))
         cols = int(np.prod(qnbigr.shape[:-1]))
@@ -67,9 +71,10 @@ def svd_qn_plan(qnbigl, qnbigr, qntot):
             qnl += [nl] * k
             qnr += [nr] * k
         plan = (rows, cols, [(b[2], b[3]) for b in blocks], qnl, qnr)
-        if len(_PLAN_CACHE) >= _PLAN_CACHE_MAX:
-            _PLAN_CACHE.pop(next(iter(_PLAN_CACHE)))
-        _PLAN_CACHE[key] = plan
+        with _PLAN_LOCK:
+            if len(_PLAN_CACHE) >= _PLAN_CACHE_MAX:
+                _PLAN_CACHE.pop(next(iter(_PLAN_CACHE)))
+            _PLAN_CACHE[key] = plan
     return plan
 
 

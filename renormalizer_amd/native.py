@@ -510,12 +510,15 @@ _INDEX_LOCK = threading.Lock()
 
 
 def _device_index(a, dev):
-    """Device copy of a host int64 index array, cached by content with
-    least-recently-used eviction.  Callers keep the returned tensor alive for
+    """Device copy of a host int64 index array, cached by content and stream
+    with least-recently-used eviction.  Callers keep the returned tensor alive for
     as long as a queued kernel may read it (an evicted tensor is freed when
     its last reference goes)."""
     a = np.ascontiguousarray(a, dtype=np.int64)
-    key = (str(dev), a.tobytes())
+    # per stream: an evicted tensor returns to the allocator pool of the
+    # stream it was made on, so only that stream's later work may reuse it
+    stream = torch.cuda.current_stream(dev).cuda_stream if dev.type == "cuda" else 0
+    key = (str(dev), int(stream), a.tobytes())
     with _INDEX_LOCK:
         t = _INDEX_CACHE.get(key)
         if t is not None:

@@ -10,5 +10,3 @@ def test_evolve_batch_arguments():
         evolve_batch([object(), object()], [object()], 0.1)
     with pytest.raises(ValueError):
         evolve_batch([object()], object(), 0.1, nstreams=0)
-    with pytest.raises(NotImplementedError):
-        evolve_batch([object()], object(), 0.1, nstreams=2)

@@ -1,6 +1,10 @@
 """Several independent states evolved on concurrent streams
 (renormalizer_amd.mps.batch.evolve_batch): bit-identical to one-by-one
-evolution, on instances of the disordered-Holstein workload."""
+evolution, on instances of the disordered-Holstein workload.  An earlier
+version of this mode stopped making progress on the device: all streams
+shared one native workspace, so two streams' Krylov calls reset each
+other's state block and the convergence test never fired; each stream now
+has its own workspace (tests/test_native_cpu.py checks that)."""
 import time
 
 import numpy as np
@@ -11,12 +15,7 @@ from renormalizer_amd import EvolveConfig, EvolveMethod
 from renormalizer_amd.mps.batch import evolve_batch
 from tests import helpers
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the concurrent-stream batch (nstreams > 1) printed nothing for 180 s in "
-                            "its first device run and was killed as hung; no cause found by reading "
-                            "the code, so the mode raises until one is shown"),
-]
+pytestmark = pytest.mark.gpu
 
 
 def _instances(sizes, seed=7):
