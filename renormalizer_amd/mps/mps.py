@@ -684,6 +684,13 @@ class Mps(MatrixProduct):
         _, s_array = mps.compress(temp_m_trunc=np.iinfo(np.int32).max, ret_s=True)
         return s_array
 
+    def calc_entropy(self, entropy_type: str):
+        """reference mps.py:1689-1757 for entropy_type "bond" (the only one
+        on this path): the von Neumann entropy of every bond."""
+        if entropy_type != "bond":
+            raise NotImplementedError(f"entropy type {entropy_type!r}: only 'bond' is provided")
+        return self.calc_bond_entropy()
+
     def calc_bond_entropy(self, s_array: np.ndarray = None) -> np.ndarray:
         """Von Neumann entropy -sum p ln p, p = s^2 / sum s^2, on every bond
         (reference mps.py:1775-1793, utils.py:41-48)."""
