@@ -68,7 +68,7 @@ def sigma_z(mps):
 PROF_PERIOD = 16  # event-time a pseudo-random 1 in 16 of the GEMM launches
 
 
-def run_mode(mps0, mpo, precision, dt, warmup, steps, profile):
+def run_mode(mps0, mpo, precision, dt, warmup, steps, profile, tau=None):
     """Evolve a copy of mps0 by warmup + steps time steps.  Each timed step is
     bracketed by device synchronisations and only the evolution itself is
     timed; <sigma_z> is taken after each step outside the timed region, and
@@ -78,6 +78,8 @@ def run_mode(mps0, mpo, precision, dt, warmup, steps, profile):
     from renormalizer_amd import EvolveConfig, EvolveMethod, Mps, native
     mps = mps0.copy()
     mps.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps, precision=precision)
+    if tau is not None:
+        mps.evolve_config.mixed_tau = tau
     mps.to_device()
     sz = [sigma_z(mps)]
     redo0 = Mps.krylov_redo_count
@@ -277,6 +279,63 @@ def step_flop(counts, shapes, mpo_shapes):
     return total
 
 
+def cpu_trajectory(workload, steps):
+    """The numpy oracle alone on one of the workloads, in this process (its
+    BLAS thread count was fixed before numpy was imported): median wall
+    time per time step after one warm-up step, initial state excluded."""
+    import numpy as np
+
+    from oracle import tdvp as orc
+    from tests import helpers
+    if workload == "c1":
+        model = helpers.sbm_model(16, 8, alpha=0.5, omega_c=5.0)
+        mps, mpo = helpers.sbm_initial(model, 16)
+        dt = 0.05
+    elif workload == "c2":
+        model = helpers.holstein_model(16, 10, j=-1.0, g=1.0)
+        mps, mpo = helpers.holstein_initial(model, 64)
+        dt = 0.02
+    elif workload == "c4":
+        _, mpo, mps = build_workload(64, 10, 256)
+        dt = 0.025
+    else:
+        raise SystemExit(f"unknown workload {workload}")
+    st = helpers.to_state(mps)
+    w = helpers.mpo_arrays(mpo)
+    times = []
+    for k in range(steps + 1):
+        t0 = time.perf_counter()
+        st, _ = orc.tdvp_ps(st, w, dt)
+        if k:
+            times.append(time.perf_counter() - t0)
+        log(f"cpu {workload}: step {k} {time.perf_counter() - t0:.2f} s")
+    med = float(np.median(times))
+    out = {"workload": workload, "steps_timed": steps, "median_step_s": round(med, 4),
+           "sweeps_per_s": round(1.0 / med, 5), "host": host_info()}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_sweep(cores):
+    """cpu_trajectory of C1 (100 steps) and C2 (5 steps) at 1, 4 and all
+    threads and of C4 (1 step) at all threads, each in a fresh process."""
+    import subprocess
+    runs = []
+    for threads in sorted({1, min(4, cores), cores}):
+        for wl, n in (("c1", 100), ("c2", 5)):
+            runs.append((wl, n, threads))
+    runs.append(("c4", 1, cores))
+    for wl, n, threads in runs:
+        env = dict(os.environ, RENO_NUM_THREADS=str(threads))
+        for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
+            env[k] = str(threads)
+        log(f"cpu sweep: {wl}, {n} steps, {threads} threads")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-trajectory", wl,
+                            "--steps", str(n)], env=env, stdout=subprocess.PIPE, text=True)
+        if r.returncode != 0:
+            raise SystemExit(f"cpu trajectory {wl} at {threads} threads failed ({r.returncode})")
+        print(r.stdout.strip().splitlines()[-1], flush=True)
+
+
 def copy_calibration(mib=512, reps=3):
     """A plain device-to-device copy of known size, for scaling the PMC
     FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -346,9 +405,15 @@ def main():
     ap.add_argument("--bond-dim", type=int, default=256)
     ap.add_argument("--modes", type=int, default=64)
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--mixed-tau", type=float, default=None,
+                    help="switch threshold of the mixed mode (EvolveConfig.mixed_tau)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="run one precision mode only (profiling runs)")
+    ap.add_argument("--cpu-trajectory", choices=["c1", "c2", "c4"],
+                    help="time the numpy oracle alone on one workload (no GPU)")
+    ap.add_argument("--cpu-sweep", action="store_true",
+                    help="cpu-trajectory of C1/C2 at 1, 4 and all threads and C4 at all threads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 512 MiB device copy (PMC counter calibration)")
     ap.add_argument("--summarize-counters", metavar="DIR",
@@ -364,6 +429,12 @@ def main():
         return
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.cpu_trajectory:
+        cpu_trajectory(a.cpu_trajectory, a.steps)
+        return
+    if a.cpu_sweep:
+        cpu_sweep(_CORES)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -385,7 +456,7 @@ def main():
         if a.only and prec != a.only:
             continue
         log(f"running {prec}: {a.warmup} warmup + {a.steps} timed steps")
-        res[prec] = run_mode(mps0, mpo, prec, a.dt, a.warmup, a.steps, profile=True)
+        res[prec] = run_mode(mps0, mpo, prec, a.dt, a.warmup, a.steps, profile=True, tau=a.mixed_tau)
         log(f"{prec}: {res[prec]['steps_per_s']:.3f} steps/s, matvecs/step "
             f"{res[prec]['matvecs_per_step']}")
     err = None
@@ -427,6 +498,8 @@ def main():
         out["max_abs_sigma_z_err_mixed_vs_fp64"] = err
         out["mixed_within_bound"] = err <= ACCURACY_BOUND
         out["steps_compared"] = len(res["fp64"]["sz"]) - 1
+        from renormalizer_amd import EvolveConfig
+        out["mixed_tau"] = a.mixed_tau if a.mixed_tau is not None else EvolveConfig().mixed_tau
         mv64, mvmx = sum(res["fp64"]["matvecs_per_step"]), sum(res["mixed"]["matvecs_per_step"])
         out["matvecs_fp64"] = mv64
         out["matvecs_mixed"] = mvmx
