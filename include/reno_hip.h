@@ -164,6 +164,68 @@ int reno_expm_krylov_mixed_async(const reno_hop_desc* op, const reno_hop_desc* o
                                  const void* v0, void* out, int max_matvec, int* info,
                                  void* ws, size_t ws_bytes, void* stream);
 
+/* ---- batched sweep: many small states, one launch per sweep position ------
+ * One state's site update of the one-site TDVP-PS sweep (reference
+ * mps/mps.py:1296-1397, FP64): expm(-i dt/2 H_eff) of the site tensor v0 by
+ * Lanczos with the reference's stopping rule (krylov.py:27-84) into kout; for
+ * a site that is not the last of the half-sweep, the quantum-number blocked
+ * QR (dir 1, left to right: (Ml d) x Mr, u -> site_out (Ml, d, kt), v^T ->
+ * bond (kt x Mr)) or RQ (dir 0, right to left: Ml x (d Mr), u -> bond
+ * (Ml x kt), v^T -> site_out (kt, d, Mr)) of kout as svd_qn(QR=True)
+ * computes it (svd_qn.py:99-240; blocks in the caller's order), the
+ * environment update of site_out (contract_one_site, lib.py:169-250: env_in
+ * L[i-1] -> env_out L[i], or R[i+1] -> R[i]), expm(+i dt/2 H_bond) of the
+ * bond matrix with the environments L0 / R0 (MPO bond D0) into bond_out, and
+ * the bond matrix absorbed into the neighbouring site nbr (dir 0: carry =
+ * nbr (nbr_a x Ml) bond_out; dir 1: carry = bond_out nbr (Mr x nbr_b)).
+ * All pointers are device pointers; Wl is the site MPO permuted to
+ * Wl[(dd, f), (b, e)] = W[b, dd, e, f].  scratch holds
+ * reno_batch_scratch_bytes(Ml, d, Mr, Dl, Dr, kcap) bytes; counts receives
+ * the matvecs of the two Krylov calls (-1: more than kcap were needed, the
+ * outputs are void; -2: zero vector). */
+typedef struct {
+  int active, dir, last;
+  int Ml, d, Mr, Dl, Dr;
+  int kt, nblocks;
+  int nbr_a, nbr_b;
+  int D0, kcap;
+  int pad0, pad1;
+  const void* L;
+  const void* R;
+  const void* Wl;
+  const void* W;
+  const void* v0;
+  void* kout;
+  void* site_out;
+  void* bond;
+  const void* env_in;
+  void* env_out;
+  const void* L0;
+  const void* R0;
+  void* bond_out;
+  const void* nbr;
+  void* carry;
+  const int64_t* brows[4];
+  const int64_t* bcols[4];
+  int bnr[4];
+  int bnc[4];
+  void* scratch;
+  int* counts;
+} reno_batch_site;
+
+size_t reno_batch_scratch_bytes(int Ml, int d, int Mr, int Dl, int Dr, int kcap);
+/* No reference counterpart (the reference evolves one state at a time):
+ * the site updates of nstates states at one sweep position, one workgroup
+ * per state, in a single launch; sites is a device array of descriptors
+ * (inactive entries do nothing).  Replaces, per state, the hop_expr /
+ * expm_krylov / svd_qn / contract_one_site sequence of mps.py:1296-1397. */
+int reno_batch_site_step(int nstates, const reno_batch_site* sites, double dt, void* stream);
+/* Scale each state's centre site (device pointers, len[b] complex128
+ * entries) to unit Frobenius norm, the norm into norms[b] (device): the
+ * normalize("mps_only") of a canonical state (mps.py:2025-2059). */
+int reno_batch_normalize(int nstates, void* const* centre, const int64_t* len, double* norms,
+                         void* stream);
+
 /* ---- svd_qn QR / RQ branch (replaces svd_qn.py:99-240, QR=True,
  * full_matrices=False) ------------------------------------------------
  * coef is the complex128 (rows x cols) coefficient matrix.  For each block

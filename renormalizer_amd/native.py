@@ -34,6 +34,7 @@ EXPORTS = (
     "reno_env_workspace_bytes", "reno_qr_workspace_bytes",
     "reno_hop_apply", "reno_hop_apply_noalloc", "reno_env_update", "reno_env_close", "reno_expm_krylov",
     "reno_expm_krylov_async", "reno_expm_krylov_mixed_async", "reno_block_qr", "reno_heff_diag", "reno_heff_dense", "reno_zgemm",
+    "reno_batch_scratch_bytes", "reno_batch_site_step", "reno_batch_normalize",
     "reno_probe_mfma16", "reno_prof_enable", "reno_prof_collect", "reno_prof_kind_info",
     "reno_prof_cancelled",
 )
@@ -127,6 +128,12 @@ def load_library(path: str = LIB_PATH):
         lib.reno_env_close.argtypes = [c_int, c_int, c_int, c_int, vp, vp, vp, vp]
         lib.reno_zgemm.restype = c_int
         lib.reno_zgemm.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp]
+        lib.reno_batch_scratch_bytes.restype = c_size
+        lib.reno_batch_scratch_bytes.argtypes = [c_int] * 6
+        lib.reno_batch_site_step.restype = c_int
+        lib.reno_batch_site_step.argtypes = [c_int, vp, c_double, vp]
+        lib.reno_batch_normalize.restype = c_int
+        lib.reno_batch_normalize.argtypes = [c_int, vp, vp, vp, vp]
         lib.reno_probe_mfma16.restype = c_int
         lib.reno_probe_mfma16.argtypes = [c_int, vp, vp, vp, vp]
         lib.reno_prof_enable.restype = c_int
