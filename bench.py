@@ -336,6 +336,123 @@ def cpu_sweep(cores):
         print(r.stdout.strip().splitlines()[-1], flush=True)
 
 
+def _c3_instance(args):
+    """One instance of the batched disorder workload (host, in a worker
+    process): Holstein chain of nmol molecules (scheme 2, 8 levels, omega 1,
+    g 0.7, J -1), site energies from N(0, 0.5), Franck-Condon electron in the
+    middle, expanded to bond dimension m and canonicalised."""
+    nmol, m, elocal = args
+    from threadpoolctl import threadpool_limits
+
+    from tests import helpers
+    with threadpool_limits(1):
+        model = helpers.holstein_model(nmol, 8, j=-1.0, g=0.7, elocal=list(elocal))
+        mps, mpo = helpers.holstein_initial(model, m)
+    return model, mps, mpo
+
+
+def c3_site_flop(local_steps, shapes, w_shapes, to_right):
+    """Algorithmic FLOP of one grouped time step of one state, from its
+    local_steps (per half-sweep and site: one-site matvecs, then the bond
+    matvecs) counted like site_update_flop."""
+    n = len(shapes)
+    first = list(range(n)) if to_right else list(range(n - 1, -1, -1))
+    it = iter(local_steps)
+    total = 0.0
+    for order, right in ((first, to_right), (first[::-1], not to_right)):
+        for p, i in enumerate(order):
+            ml, d, mr = shapes[i]
+            dl, dr = w_shapes[i][0], w_shapes[i][3]
+            f_hop1 = 8 * (ml * ml * dl * d * mr + ml * d * dr * mr * mr) + 4 * ml * mr * dl * dr * d * d
+            total += next(it) * f_hop1
+            if p == n - 1:
+                continue
+            mb, db = (mr, dr) if right else (ml, dl)
+            k0 = next(it)
+            qr = qr_flop(ml * d, mr) if right else qr_flop(d * mr, ml)
+            total += k0 * 8 * db * mb * mb * 2 * mb + qr + f_hop1 + 8 * mb * mb * mb * d
+    return total
+
+
+def run_c3(instances, steps, warmup, seed=2026, dt=0.02):
+    """The batched disorder workload: every state's site update at one sweep
+    position in one launch (BatchEvolution), against the same states evolved
+    one at a time on four concurrent streams (evolve_batch)."""
+    import multiprocessing as mp_
+
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    args = []
+    for i in range(instances):
+        nmol = int(rng.integers(8, 33))
+        args.append((nmol, 16 * 2 ** (i % 3), rng.normal(0.0, 0.5, nmol)))
+    t0 = time.perf_counter()
+    ctx = mp_.get_context("fork")
+    with ctx.Pool(min(_CORES, 16)) as pool:
+        built = pool.map(_c3_instance, args, chunksize=1)
+    log(f"c3: {instances} initial states on the host in {time.perf_counter() - t0:.1f} s")
+    import torch
+    from renormalizer_amd import EvolveConfig, EvolveMethod
+    from renormalizer_amd.mps.batch import BatchEvolution, evolve_batch
+    torch.cuda.set_device(0)
+    states = []
+    for _, mps, _ in built:
+        mps.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps)
+        mps.to_device()
+        states.append(mps)
+    mpos = [o for _, _, o in built]
+    t0 = time.perf_counter()
+    be = BatchEvolution(states, mpos, dt, warmup=max(1, warmup), nstreams=4)
+    torch.cuda.synchronize()
+    log(f"c3: warm-up (one-state path, 4 streams) and grouped set-up {time.perf_counter() - t0:.1f} s")
+    # reference rate: one step of every state through the one-state path
+    ref = [m.copy() for m in be.states]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref = evolve_batch(ref, mpos, dt, nstreams=4)
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    del ref
+    be.step()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    flop = 0.0
+    pops = []
+    for _ in range(steps):
+        be.step()
+        for m, ls, o in zip(be.states, be.local_steps, mpos):
+            flop += c3_site_flop(ls, [tuple(t.shape) for t in m._mp], [np.asarray(w).shape for w in o],
+                                 m.to_right)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_grouped = (time.perf_counter() - t0) / steps
+    gpu_ms = ev0.elapsed_time(ev1) / steps
+    launches = 2 * be.npos + 1
+    pops = float(np.mean([m.e_occupations.mean() for m in be.states[:8]]))
+    out = {
+        "metric": "batched disorder: instance time steps per second",
+        "value": round(instances / t_grouped, 2), "unit": "instance-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(1e3 * t_grouped, 2),
+        "higher_is_better": True, "dtype": "c128", "data": "synthetic",
+        "config": {"workload": f"C3: {instances} Holstein chains of 8..32 molecules (scheme 2, 8 levels, "
+                               f"omega 1, g 0.7, J -1, site energies N(0, 0.5), seed {seed}), "
+                               f"M = 16/32/64, dt {dt}, TDVP-PS one-site, FP64",
+                   "launches_per_step": launches},
+        "one_state_path_4_streams_ms_per_step": round(1e3 * t_single, 1),
+        "speedup_vs_one_state_path": round(t_single / t_grouped, 2),
+        "roofline": {"bound": "mfma", "achieved": round(flop / steps / (gpu_ms * 1e-3) / 1e12, 3),
+                     "peak": PEAK_TFLOPS["double"], "unit": "TFLOP/s",
+                     "frac": round(flop / steps / (gpu_ms * 1e-3) / 1e12 / PEAK_TFLOPS["double"], 4),
+                     "traffic": None, "kernel": "k_batch_site(reno_batch_site const*, double)",
+                     "flop_per_step": flop / steps, "gpu_ms_per_step": round(gpu_ms, 2),
+                     "avg_launch_us": round(1e3 * gpu_ms / launches, 1)},
+        "mean_population_check": pops,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration(mib=512, reps=3):
     """A plain device-to-device copy of known size, for scaling the PMC
     FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -410,6 +527,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="run one precision mode only (profiling runs)")
+    ap.add_argument("--workload", choices=["c4", "c3"], default="c4",
+                    help="c4: the metric workload (default); c3: the batched disorder workload")
+    ap.add_argument("--instances", type=int, default=256, help="c3: number of states")
     ap.add_argument("--cpu-trajectory", choices=["c1", "c2", "c4"],
                     help="time the numpy oracle alone on one workload (no GPU)")
     ap.add_argument("--cpu-sweep", action="store_true",
@@ -435,6 +555,9 @@ def main():
         return
     if a.cpu_sweep:
         cpu_sweep(_CORES)
+        return
+    if a.workload == "c3":
+        run_c3(a.instances, a.steps, a.warmup)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:

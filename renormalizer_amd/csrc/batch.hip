@@ -23,7 +23,7 @@
 namespace reno {
 
 constexpr int kBT = 256;      // threads per workgroup
-constexpr int kBTile = 32;    // GEMM output tile (four waves of 16 x 16)
+constexpr int kBTile = 64;    // GEMM output tile (four waves of 32 x 32)
 constexpr int kBK = 16;       // GEMM K step
 
 typedef double d4b_t __attribute__((ext_vector_type(4)));
@@ -57,11 +57,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // C[b] (+)= op(A[b]) op(B[b]) for one GemmCall (the strided batched product
 // of reno_common.h; A complex128 or float64, B and C complex128), computed by
-// the calling workgroup: 32 x 32 output tiles in turn, K in steps of 16
-// staged through LDS, each wave one 16 x 16 block of
-// v_mfma_f64_16x16x4f64 (three real products per complex product, two when
-// A is real).  MFMA 16x16x4 f64 lane layout: A lane l = A[l % 16][l / 16],
-// B lane l = B[l / 16][l % 16], D lane l register r = D[4 r + l / 16][l % 16].
+// the calling workgroup: 64 x 64 output tiles in turn, K in steps of 16
+// staged through LDS with the next step's elements already loaded into
+// registers while the current one is multiplied; each wave owns a 32 x 32
+// quadrant of 2 x 2 v_mfma_f64_16x16x4f64 blocks (three real products per
+// complex product, two when A is real).  MFMA 16x16x4 f64 lane layout:
+// A lane l = A[l % 16][l / 16], B lane l = B[l / 16][l % 16], D lane l
+// register r = D[4 r + l / 16][l % 16].
 __device__ void wg_gemm(const GemmCall& g, BatchLds& s) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -72,18 +74,19 @@ __device__ void wg_gemm(const GemmCall& g, BatchLds& s) {
   const double2* B = (const double2*)g.B;
   double2* C = (double2*)g.C;
   const double sa = g.conjA ? -1.0 : 1.0, sb = g.conjB ? -1.0 : 1.0;
+  constexpr int kPer = kBTile * kBK / kBT;  // elements of each operand per thread
   __syncthreads();  // the operands may have been written by this workgroup
   const int64_t ntile = (int64_t)g.batch * tm * tn;
   for (int64_t bt = 0; bt < ntile; ++bt) {
     const int64_t b = bt / (tm * tn);
     const int t = (int)(bt - b * tm * tn);
     const int m0 = (t / tn) * kBTile, n0 = (t % tn) * kBTile;
-    d4b_t cr = {0.0, 0.0, 0.0, 0.0}, ci = cr, cq = cr;
-    for (int k0 = 0; k0 < g.K; k0 += kBK) {
+    double pa_r[kPer], pa_i[kPer], pb_r[kPer], pb_i[kPer];
+    auto fetch = [&](int k0) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < kPer; ++i) {
         const int e = tid + kBT * i;
-        const int x = e & (kBTile - 1), kk = e >> 5;
+        const int x = e % kBTile, kk = e / kBTile;
         double re = 0.0, im = 0.0;
         if (m0 + x < g.M && k0 + kk < g.K) {
           const int64_t off = b * g.sAb + (int64_t)(m0 + x) * g.sAm + (int64_t)(k0 + kk) * g.sAk;
@@ -95,8 +98,8 @@ __device__ void wg_gemm(const GemmCall& g, BatchLds& s) {
             im = sa * v.y;
           }
         }
-        s.ta[0][kk][x] = re;
-        s.ta[1][kk][x] = im;
+        pa_r[i] = re;
+        pa_i[i] = im;
         re = 0.0;
         im = 0.0;
         if (n0 + x < g.N && k0 + kk < g.K) {
@@ -104,44 +107,84 @@ __device__ void wg_gemm(const GemmCall& g, BatchLds& s) {
           re = v.x;
           im = sb * v.y;
         }
-        s.tb[0][kk][x] = re;
-        s.tb[1][kk][x] = im;
+        pb_r[i] = re;
+        pb_i[i] = im;
       }
-      __syncthreads();
+    };
+    auto stash = [&]() {
+#pragma unroll
+      for (int i = 0; i < kPer; ++i) {
+        const int e = tid + kBT * i;
+        const int x = e % kBTile, kk = e / kBTile;
+        s.ta[0][kk][x] = pa_r[i];
+        s.ta[1][kk][x] = pa_i[i];
+        s.tb[0][kk][x] = pb_r[i];
+        s.tb[1][kk][x] = pb_i[i];
+      }
+    };
+    d4b_t cr[2][2], ci[2][2], cq[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) cr[i][j] = ci[i][j] = cq[i][j] = (d4b_t){0.0, 0.0, 0.0, 0.0};
+    fetch(0);
+    stash();
+    __syncthreads();
+    for (int k0 = 0; k0 < g.K; k0 += kBK) {
+      const bool more = k0 + kBK < g.K;
+      if (more) fetch(k0 + kBK);
 #pragma unroll
       for (int ks = 0; ks < kBK / 4; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
-        const double ar = s.ta[0][kr][wm * 16 + (lane & 15)];
-        const double ai = s.ta[1][kr][wm * 16 + (lane & 15)];
-        const double br = s.tb[0][kr][wn * 16 + (lane & 15)];
-        const double bi = s.tb[1][kr][wn * 16 + (lane & 15)];
-        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cr, 0, 0, 0);
-        if (areal) {
-          ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ci, 0, 0, 0);
-        } else {
-          cq = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, cq, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, br + bi, ci, 0, 0, 0);
+        double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          ar[q] = s.ta[0][kr][wm * 32 + q * 16 + (lane & 15)];
+          ai[q] = s.ta[1][kr][wm * 32 + q * 16 + (lane & 15)];
+          br[q] = s.tb[0][kr][wn * 32 + q * 16 + (lane & 15)];
+          bi[q] = s.tb[1][kr][wn * 32 + q * 16 + (lane & 15)];
         }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], cr[i][j], 0, 0, 0);
+            if (areal) {
+              ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], ci[i][j], 0, 0, 0);
+            } else {
+              cq[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], cq[i][j], 0, 0, 0);
+              ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i] + ai[i], br[j] + bi[j], ci[i][j], 0,
+                                                                0, 0);
+            }
+          }
       }
       __syncthreads();
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = m0 + wm * 16 + 4 * r + (lane >> 4);
-      const int col = n0 + wn * 16 + (lane & 15);
-      if (row < g.M && col < g.N) {
-        const double re = areal ? cr[r] : cr[r] - cq[r];
-        const double im = areal ? ci[r] : ci[r] - cr[r] - cq[r];
-        double2* p = C + b * g.sCb + (int64_t)row * g.sCm + col;
-        double2 v = make_double2(re, im);
-        if (g.beta) {
-          const double2 o = *p;
-          v.x += o.x;
-          v.y += o.y;
-        }
-        *p = v;
+      if (more) {
+        stash();
+        __syncthreads();
       }
     }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * 32 + i * 16 + 4 * r + (lane >> 4);
+          const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+          if (row < g.M && col < g.N) {
+            const double re = areal ? cr[i][j][r] : cr[i][j][r] - cq[i][j][r];
+            const double im = areal ? ci[i][j][r] : ci[i][j][r] - cr[i][j][r] - cq[i][j][r];
+            double2* p = C + b * g.sCb + (int64_t)row * g.sCm + col;
+            double2 v = make_double2(re, im);
+            if (g.beta) {
+              const double2 o = *p;
+              v.x += o.x;
+              v.y += o.y;
+            }
+            *p = v;
+          }
+        }
   }
   __syncthreads();
 }
