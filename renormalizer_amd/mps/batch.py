@@ -288,10 +288,7 @@ class BatchEvolution:
         self.centre_len = torch.tensor(lens, dtype=torch.int64, device=dev)
         self.norms = torch.empty(nst, dtype=torch.float64, device=dev)
 
-    def step(self):
-        """One time step of every state: 2 x (longest chain) launches, the
-        centre sites normalised (normalize("mps_only")), one wait for the
-        matvec counts at the end."""
+    def _launch_step(self):
         native = self.native
         lib = native.lib()
         stream = native._stream()
@@ -304,6 +301,32 @@ class BatchEvolution:
                                               ctypes.c_void_p(self.centre_len.data_ptr()),
                                               ctypes.c_void_p(self.norms.data_ptr()), stream),
                      "reno_batch_normalize")
+
+    def capture(self):
+        """Record one time step (all launches of both half-sweeps and the
+        normalisation) as a HIP graph; ``step`` then replays it.  Every
+        operand lives at a fixed address and the step allocates nothing and
+        clears nothing, so each replay advances the states by one step."""
+        stream = torch.cuda.Stream(device=self.device)
+        stream.wait_stream(torch.cuda.current_stream(self.device))
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            self._launch_step()
+        torch.cuda.current_stream(self.device).wait_stream(stream)
+        self.graph = graph
+        return self
+
+    graph = None
+
+    def step(self):
+        """One time step of every state: 2 x (longest chain) launches (or one
+        graph replay after ``capture``), the centre sites normalised
+        (normalize("mps_only")), one wait for the matvec counts at the end."""
+        native = self.native
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self._launch_step()
         counts = self.counts.cpu().numpy()
         if (counts < 0).any():
             bad = np.argwhere(counts < 0)[0]

@@ -14,6 +14,7 @@ Elsewhere the allclose test of an element at its threshold may flip the
 same way, on at most 3 % of the local steps as in tests/test_tdvp_gpu.py."""
 import numpy as np
 import pytest
+import torch
 
 from oracle import tdvp as orc
 from renormalizer_amd import EvolveConfig, EvolveMethod
@@ -122,3 +123,31 @@ def test_grouped_spin_boson_vs_oracle():
     worst = _run(inst, sz_dev, sz_orc, 0.05, 3)
     print(f"[grouped sbm] max |dsz| vs oracle {worst:.2e}", flush=True)
     assert worst < 1e-8
+
+
+def test_grouped_step_as_graph_replay():
+    """The grouped time step recorded once as a HIP graph and replayed four
+    times, with eager work and host waits between the replays, gives the
+    eager steps' states bit for bit after every replay."""
+    print("[grouped sweep: graph replay vs eager, 6 chains]", flush=True)
+    inst = _disorder_instances(6, 9)
+    runs = []
+    for _ in range(2):
+        states = []
+        for model, mps, mpo in inst:
+            m = mps.copy()
+            m.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps)
+            m.to_device()
+            states.append(m)
+        runs.append(BatchEvolution(states, [o for _, _, o in inst], 0.02, warmup=1))
+    eager, graphed = runs
+    graphed.capture()
+    for k in range(4):
+        eager.step()
+        graphed.step()
+        occ = [m.e_occupations for m in eager.states]   # eager work between replays
+        for a, b in zip(eager.states, graphed.states):
+            for x, y in zip(a._mp, b._mp):
+                assert torch.equal(x, y), k
+        assert eager.local_steps == graphed.local_steps
+        assert all(np.isfinite(o).all() for o in occ)
