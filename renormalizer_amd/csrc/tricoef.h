@@ -33,10 +33,11 @@ __device__ __forceinline__ double wave_shl1(double v) {
 constexpr int kChebMax = 512;  // Bessel terms of the Chebyshev branch
 
 // J_k(x), k = 0 .. nmax - 1, for x > 0 by Miller's backward recurrence
-// J_{k-1} = (2k / x) J_k - J_{k+1} from a start well above nmax, normalised
-// by J_0 + 2 sum_{k>=1} J_{2k} = 1 (one thread).  Returns the number of
-// leading terms to keep: past the last one every |J_k| < 1e-18.
-__device__ inline int bessel_j_miller(double x, int nmax, double* J) {
+// J_{k-1} = (2k / x) J_k - J_{k+1} from a start well above nmax (one
+// thread).  Stores the unnormalised values in J[0..nmax) and the
+// normalisation sum J_0 + 2 sum_{k>=1} J_{2k} in J[nmax]; the caller's wave
+// divides by it (J holds nmax + 1 doubles).
+__device__ inline void bessel_j_miller(double x, int nmax, double* J) {
   const int start = nmax + 24 + (nmax & 1);
   const double ix = 1.0 / x;
   double jp1 = 0.0, jk = 1e-30, norm = 0.0;
@@ -54,13 +55,12 @@ __device__ inline int bessel_j_miller(double x, int nmax, double* J) {
       for (int i = k - 1; i < nmax; ++i) J[i] *= 1e-200;
     }
   }
-  const double in = 1.0 / norm;
-  int keep = 1;
-  for (int k = 0; k < nmax; ++k) {
-    J[k] *= in;
-    if (fabs(J[k]) >= 1e-18) keep = k + 1;
-  }
-  return keep;
+  J[nmax] = norm;
+}
+
+__device__ __forceinline__ int wave_max_int(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
 }
 
 // coef = |v0| expm(dt T) e_0 for the m x m tridiagonal Lanczos matrix T
@@ -69,14 +69,15 @@ __device__ inline int bessel_j_miller(double x, int nmax, double* J) {
 // * real-time steps (dt = i tau, the TDVP path): the Chebyshev expansion
 //   exp(i w y) = J_0(w) + 2 sum_{k>=1} i^k J_k(w) T_k(y), w = tau r, on
 //   Y e_0, whose Chebyshev vectors T_k(Y) e_0 are real (three-term
-//   recurrence); J_k from bessel_j_miller, about w + 25 terms;
+//   recurrence); J_k from bessel_j_miller (normalised by the wave), about
+//   w + 25 terms;
 // * other dt: exp(dt c) [Taylor_K(h (T - c I))]^q e_0 with q = ceil(|dt| r / 4),
 //   h = dt / q: every factor has ||h (T - c I)|| <= 4, and K = 34 terms leave
 //   a truncation error below 4^35 / 35! ~ 1e-19.
 // One wavefront (lanes threadIdx.x & 63); lane l holds entries
 // l*R .. l*R+R-1; neighbours cross lanes by DPP wave shifts.  Fixed
-// operation order.  Every thread of the workgroup must call it (it holds a
-// workgroup barrier); sJ (kChebMax doubles) and sK are the calling wave's
+// operation order.  Every thread of the workgroup must call it (it holds
+// workgroup barriers); sJ (kChebMax doubles) and sK are the calling wave's
 // LDS, coef (m entries) is written unless null.
 template <int R>
 __device__ __forceinline__ void tri_coef_wave(const double* __restrict__ alpha,
@@ -108,7 +109,21 @@ __device__ __forceinline__ void tri_coef_wave(const double* __restrict__ alpha,
   const double x = fabs(w);
   const int nterm = (int)(x + 12.0 * cbrt(x) + 32.0);
   if (dt_re == 0.0 && x >= 0.05 && nterm < kChebMax - 24) {
-    if (lane == 0) *sK = bessel_j_miller(x, nterm, sJ);
+    if (lane == 0) bessel_j_miller(x, nterm, sJ);
+    __syncthreads();
+    // normalise across the wave; K = number of leading terms to keep (past
+    // the last one every |J_k| < 1e-18)
+    {
+      const double in = 1.0 / sJ[nterm];
+      int keep = 1;
+      for (int k = lane; k < nterm; k += 64) {
+        const double v = sJ[k] * in;
+        sJ[k] = v;
+        if (fabs(v) >= 1e-18) keep = k + 1;
+      }
+      keep = wave_max_int(keep);
+      if (lane == 0) *sK = keep;
+    }
     __syncthreads();
     const int K = *sK;
     const double ir = 1.0 / rad;
