@@ -68,7 +68,7 @@ def sigma_z(mps):
 PROF_PERIOD = 16  # event-time a pseudo-random 1 in 16 of the GEMM launches
 
 
-def run_mode(mps0, mpo, precision, dt, warmup, steps, profile, tau=None):
+def run_mode(mps0, mpo, precision, dt, warmup, steps, profile, tau=None, autotune=False):
     """Evolve a copy of mps0 by warmup + steps time steps.  Each timed step is
     bracketed by device synchronisations and only the evolution itself is
     timed; <sigma_z> is taken after each step outside the timed region, and
@@ -80,6 +80,7 @@ def run_mode(mps0, mpo, precision, dt, warmup, steps, profile, tau=None):
     mps.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps, precision=precision)
     if tau is not None:
         mps.evolve_config.mixed_tau = tau
+    mps.evolve_config.autotune = autotune
     mps.to_device()
     sz = [sigma_z(mps)]
     redo0 = Mps.krylov_redo_count
@@ -119,21 +120,23 @@ def run_mode(mps0, mpo, precision, dt, warmup, steps, profile, tau=None):
             "matvecs_per_step": matvecs, "fp32_matvecs_per_step": low, "final": mps, "environ": env, "redone": redone}
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tdvp_c4_m256_fp64_hbm_traffic.md")
+TRAFFIC_PROFILES = [os.path.join(ROOT, "profiles", f"tdvp_c4_m256_{mode}_hbm_traffic.md")
+                    for mode in ("fp64", "mixed")]
 
 
 def measured_traffic(kernel):
     """HBM bytes per executed launch of ``kernel`` from the committed
-    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (lines
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summaries (lines
     "traffic_bytes_per_launch | <kernel> | <bytes>"), or None."""
-    try:
-        with open(TRAFFIC_PROFILE) as f:
-            for line in f:
-                parts = [p.strip() for p in line.strip().strip("|").split("|")]
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-                    return float(parts[2])
-    except OSError:
-        pass
+    for path in TRAFFIC_PROFILES:
+        try:
+            with open(path) as f:
+                for line in f:
+                    parts = [p.strip() for p in line.strip().strip("|").split("|")]
+                    if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                        return float(parts[2])
+        except OSError:
+            pass
     return None
 
 
@@ -525,6 +528,9 @@ def main():
     ap.add_argument("--mixed-tau", type=float, default=None,
                     help="switch threshold of the mixed mode (EvolveConfig.mixed_tau)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
+                    help="time the tile / split-K candidates of each matvec shape during the "
+                         "warm-up (EvolveConfig.autotune) and use the fastest")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="run one precision mode only (profiling runs)")
     ap.add_argument("--workload", choices=["c4", "c3"], default="c4",
@@ -579,9 +585,16 @@ def main():
         if a.only and prec != a.only:
             continue
         log(f"running {prec}: {a.warmup} warmup + {a.steps} timed steps")
-        res[prec] = run_mode(mps0, mpo, prec, a.dt, a.warmup, a.steps, profile=True, tau=a.mixed_tau)
+        res[prec] = run_mode(mps0, mpo, prec, a.dt, a.warmup, a.steps, profile=True, tau=a.mixed_tau,
+                             autotune=a.autotune)
         log(f"{prec}: {res[prec]['steps_per_s']:.3f} steps/s, matvecs/step "
             f"{res[prec]['matvecs_per_step']}")
+    if a.autotune:
+        from renormalizer_amd import native
+        for t in native.tuned_records:
+            log(f"autotune: prec {t['prec']} M {t['M']} N {t['N']} K {t['K']} batch {t['batch']}: "
+                f"tile {t['tile']} split {t['nsplit']} {t['us_tuned']:.1f} us (planner: tile "
+                f"{t['planned_tile']} split {t['planned_nsplit']} {t['us_planned']:.1f} us)")
     err = None
     if "fp64" in res and "mixed" in res:
         import numpy as np
@@ -610,7 +623,8 @@ def main():
         "data": "synthetic (model parameters from the workload definition; no checkpoints)",
         "config": {"workload": f"spin-boson 1 spin + {a.modes} modes x 10 levels, Ohmic alpha=0.3 "
                                f"omega_c=10 Wang1, M={a.bond_dim}, dt={a.dt}, TDVP-PS one-site",
-                   "mode": main_mode, "sweep": "1 step = 2 half-sweeps"},
+                   "mode": main_mode, "sweep": "1 step = 2 half-sweeps",
+                   "autotune": bool(a.autotune)},
         "roofline": rl,
         "krylov_steps_redone": r["redone"],
     }

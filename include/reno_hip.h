@@ -94,6 +94,35 @@ int reno_hop_apply(const reno_hop_desc* op, const void* c_in, void* c_out,
 int reno_hop_apply_noalloc(const reno_hop_desc* op, const void* c_in, void* c_out,
                            void* stream);
 
+/* ---- tile autotuner ------------------------------------------------------
+ * No code counterpart in the reference (its contractions go through
+ * opt_einsum / cuBLAS, oe_contract_wrap.py:24-58, which pick their own
+ * kernels); the proposed reno_autotune of the design.  Runs the matvec of
+ * `op` once (c_in -> c_out, workspace as for reno_hop_apply), then times
+ * every tile / split-K candidate of each of its GEMM stages on `stream`
+ * with HIP events (`reps` launches per candidate) and keeps the fastest in
+ * a process-wide table, guarded by a mutex, that every later launch of the
+ * same GEMM signature (precision, shape, storage kinds) consults, from any
+ * entry point.  Stages already in the table are not re-timed.  Writes one
+ * record per newly tuned stage into out[0..max_out) and their number into
+ * *n_out.  Synchronises the stream.  Tuned splits change the summation
+ * order of the split-K reduction (results differ at rounding level from
+ * the untuned planner's). */
+typedef struct {
+  int M, N, K, batch, prec;
+  int tile, nsplit;                 /* chosen: tile edge 64 or 32, K chunks */
+  int planned_tile, planned_nsplit; /* the built-in planner's choice */
+  float us_tuned, us_planned;       /* measured microseconds per launch */
+} reno_tile_cfg;
+
+int reno_autotune(const reno_hop_desc* op, const void* c_in, void* c_out, void* ws,
+                  size_t ws_bytes, int reps, reno_tile_cfg* out, int max_out, int* n_out,
+                  void* stream);
+/* forget every tuned choice (the planner decides again) */
+int reno_autotune_reset(void);
+/* number of GEMM signatures in the tuned table */
+int reno_autotune_entries(void);
+
 /* ---- contract_one_site (replaces lib.py:169-250) --------------------
  * domain 0 = "L": out[f,g,h] = sum env[a,b,c] conj(ms[a,d,f]) W[b,d,e,g] ms[c,e,h]
  *   env_in (Ml, Dl, Ml) -> env_out (Mr, Dr, Mr)

@@ -341,6 +341,58 @@ extern "C" int reno_hop_apply(const reno_hop_desc* op, const void* c_in, void* c
   return hop_apply_core(op, wl, c_in, c_out, t1, t2, s, nullptr, part, h.part_bytes);
 }
 
+extern "C" int reno_autotune(const reno_hop_desc* op, const void* c_in, void* c_out, void* ws,
+                             size_t ws_bytes, int reps, reno_tile_cfg* out, int max_out,
+                             int* n_out, void* stream) {
+  if (n_out) *n_out = 0;
+  RENO_TRY(check_hop(op));
+  if (!c_in || !c_out || c_in == c_out) {
+    set_error("autotune: null or aliased vectors");
+    return RENO_EINVAL;
+  }
+  if (reps < 1 || reps > 1000) {
+    set_error("autotune: reps %d outside [1, 1000]", reps);
+    return RENO_EINVAL;
+  }
+  const HopLayout h = hop_layout(op);
+  if (ws_bytes < h.total || (h.total && !ws)) {
+    set_error("autotune: workspace %zu < %zu", ws_bytes, h.total);
+    return RENO_ENOSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  char* p = (char*)ws;
+  void* wl = p;
+  void* t1 = p + h.wl_bytes;
+  void* t2 = p + h.wl_bytes + h.t1_bytes;
+  void* part = p + h.wl_bytes + h.t1_bytes + h.t2_bytes;
+  // the stages' GEMM calls, as the matvec would launch them
+  std::vector<GemmCall> calls;
+  gemm_record_begin(&calls);
+  const int rec = hop_apply_core(op, wl, c_in, c_out, t1, t2, s, nullptr, part, h.part_bytes);
+  gemm_record_end();
+  RENO_TRY(rec);
+  bool all = true;
+  for (const GemmCall& g : calls) all = all && gemm_is_tuned(g);
+  if (all) return RENO_OK;
+  // one real matvec first, so every stage reads its true operands
+  if (op->nsite == 1) RENO_TRY(permute_w(op->prec, op->W, wl, op->Dl, op->d, op->Dr, s));
+  RENO_TRY(hop_apply_core(op, wl, c_in, c_out, t1, t2, s, nullptr, part, h.part_bytes));
+  int n = 0;
+  for (const GemmCall& g : calls) {
+    if (gemm_is_tuned(g)) continue;
+    reno_tile_cfg cfg{};
+    RENO_TRY(gemm_tune(g, s, reps, &cfg));
+    if (out && n < max_out) out[n] = cfg;
+    ++n;
+  }
+  if (n_out) *n_out = n < max_out ? n : max_out;
+  return RENO_OK;
+}
+
+extern "C" int reno_autotune_reset(void) { return gemm_tune_reset(); }
+
+extern "C" int reno_autotune_entries(void) { return gemm_tune_entries(); }
+
 extern "C" size_t reno_env_workspace_bytes(int prec, int Ml, int d, int Mr, int Dl, int Dr) {
   return env_ws_bytes(prec, Ml, d, Mr, Dl, Dr);
 }

@@ -20,8 +20,12 @@
 //   C/D f64  : lane l, register i holds C[4 * i + l / 16][l % 16]
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+#include <atomic>
+#include <cstring>
 #include <mutex>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "reno_common.h"
@@ -394,6 +398,16 @@ static int find_kind(const GemmCall& g, bool ak, bool bk) {
   return any;
 }
 
+// Tile edge of the planner: 32 x 32 when the 64 x 64 grid is small (the
+// bond-matrix products of the zero-site step: four times the workgroups,
+// less split-K) or when M or N is at most 32, where a 64-wide tile would
+// leave at least half of it idle (the batched MPO stage, M = d D_r = 30 at
+// the spin-boson workload).
+static int planned_tile(int M, int N, int batch) {
+  const int64_t tiles64 = (int64_t)((M + kBM - 1) / kBM) * ((N + kBN - 1) / kBN) * batch;
+  return (tiles64 <= kSmallGridTiles || M <= kSmallTile || N <= kSmallTile) ? kSmallTile : kBM;
+}
+
 // ---- split-K -------------------------------------------------------------
 // Split K into chunks (multiples of the K tile) when the output tile grid
 // would leave the chip idle; each chunk writes its own partial and
@@ -569,6 +583,56 @@ static thread_local int t_hook_suspended = 0;
 TimingHookSuspend::TimingHookSuspend() { ++t_hook_suspended; }
 TimingHookSuspend::~TimingHookSuspend() { --t_hook_suspended; }
 
+// ---- tile autotuner state --------------------------------------------------
+// Tuned (tile, split) choices keyed by everything that selects the kernel and
+// its grid: precision, shape, batch, storage kinds, contiguity, conjugation.
+// The planner above is the default; reno_autotune measures the candidates
+// of one operator's GEMMs on the device and stores the fastest here.
+struct TuneKey {
+  int prec, M, N, K, batch, a_kind, b_kind, c_kind, ak, bk, conjA, conjB;
+  bool operator==(const TuneKey& o) const { return std::memcmp(this, &o, sizeof o) == 0; }
+};
+struct TuneKeyHash {
+  size_t operator()(const TuneKey& k) const {
+    const int* w = &k.prec;
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < sizeof k / sizeof(int); ++i) h = (h ^ (uint32_t)w[i]) * 1099511628211ull;
+    return (size_t)h;
+  }
+};
+struct TileChoice {
+  int tile, nsplit;
+};
+static std::mutex g_tune_mu;
+static std::unordered_map<TuneKey, TileChoice, TuneKeyHash> g_tuned;
+static std::atomic<int> g_tuned_count{0};
+// tuner only: a forced choice for this thread's launches, or a list that
+// receives this thread's GEMM calls instead of launching them
+static thread_local const TileChoice* t_force = nullptr;
+static thread_local std::vector<GemmCall>* t_record = nullptr;
+
+static TuneKey tune_key(const GemmCall& g) {
+  TuneKey k;
+  std::memset(&k, 0, sizeof k);
+  k.prec = g.prec; k.M = g.M; k.N = g.N; k.K = g.K; k.batch = g.batch;
+  k.a_kind = g.a_kind; k.b_kind = g.b_kind; k.c_kind = g.c_kind;
+  k.ak = g.sAk == 1; k.bk = g.sBk == 1; k.conjA = g.conjA; k.conjB = g.conjB;
+  return k;
+}
+
+// K split into (about) s chunks that are multiples of the K tile
+static void split_fixed(int K, int s, int* nsplit, int* kchunk) {
+  if (K <= 0 || s <= 1) {
+    *nsplit = 1;
+    *kchunk = K > 0 ? K : 1;
+    return;
+  }
+  int chunk = (K + s - 1) / s;
+  chunk = (chunk + kBK - 1) / kBK * kBK;
+  *kchunk = chunk;
+  *nsplit = (K + chunk - 1) / chunk;
+}
+
 int launch_gemm(const GemmCall& g, hipStream_t stream) {
   if (g.M < 0 || g.N < 0 || g.K < 0 || g.batch < 0) {
     set_error("gemm: negative size M=%d N=%d K=%d batch=%d", g.M, g.N, g.K, g.batch);
@@ -594,30 +658,51 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
     set_error("gemm: B must be complex");
     return RENO_EINVAL;
   }
-  // 32 x 32 tiles when the 64 x 64 grid is small (the bond-matrix products
-  // of the zero-site step): four times the workgroups, less split-K
-  const int64_t tiles64 = (int64_t)((g.M + kBM - 1) / kBM) * ((g.N + kBN - 1) / kBN) * g.batch;
-  const int tile = tiles64 <= kSmallGridTiles ? kSmallTile : kBM;
-  const int64_t tiles = (int64_t)((g.M + tile - 1) / tile) * ((g.N + tile - 1) / tile);
-  if (tiles > 0x7fffffff) {
-    set_error("gemm: too many tiles");
-    return RENO_EINVAL;
-  }
   if (g.dot_part && (g.batch != 1 || g.sCm != g.N || g.c_kind != ST_Z128 || !g.dot_v ||
                      g.dot_np < 1)) {
     set_error("gemm: fused dot needs one contiguous complex128 result");
     return RENO_EINVAL;
   }
-  const bool ak = (g.sAk == 1);
-  const bool bk = (g.sBk == 1);
+  if (t_record) {
+    t_record->push_back(g);
+    return RENO_OK;
+  }
+  int tile = planned_tile(g.M, g.N, g.batch);
   // split K when the tile grid cannot fill the chip and scratch is given
   int nsplit = 1, kchunk = g.K > 0 ? g.K : 1;
   split_plan(g.M, g.N, g.K, g.batch, &nsplit, &kchunk, tile);
-  const size_t need = nsplit > 1 ? (size_t)nsplit * g.batch * g.M * g.N * elem_bytes(g.prec) : 0;
-  if (nsplit > 1 && (!g.part || g.part_bytes < need)) {
+  const size_t cell = (size_t)g.batch * g.M * g.N * elem_bytes(g.prec);
+  if (nsplit > 1 && (!g.part || g.part_bytes < nsplit * cell)) {
     nsplit = 1;
     kchunk = g.K > 0 ? g.K : 1;
   }
+  // a tuned choice replaces the planner's when its partials fit the scratch
+  {
+    TileChoice ch{0, 0};
+    if (t_force) {
+      ch = *t_force;
+    } else if (g_tuned_count.load(std::memory_order_acquire) > 0) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tuned.find(tune_key(g));
+      if (it != g_tuned.end()) ch = it->second;
+    }
+    if (ch.tile) {
+      int ns, chunk;
+      split_fixed(g.K, ch.nsplit, &ns, &chunk);
+      if (ns == 1 || (g.part && g.part_bytes >= ns * cell)) {
+        tile = ch.tile;
+        nsplit = ns;
+        kchunk = chunk;
+      }
+    }
+  }
+  const int64_t tiles = (int64_t)((g.M + tile - 1) / tile) * ((g.N + tile - 1) / tile);
+  if (tiles > 0x7fffffff) {
+    set_error("gemm: too many tiles");
+    return RENO_EINVAL;
+  }
+  const bool ak = (g.sAk == 1);
+  const bool bk = (g.sBk == 1);
   GemmCall k = g;
   k.kchunk = kchunk;
   int kid = -1;
@@ -721,9 +806,102 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
 
 size_t splitk_bytes(int prec, int M, int N, int K, int batch) {
   int nsplit = 1, kchunk = 1;
-  const int64_t tiles64 = (int64_t)((M + kBM - 1) / kBM) * ((N + kBN - 1) / kBN) * batch;
-  split_plan(M, N, K, batch, &nsplit, &kchunk, tiles64 <= kSmallGridTiles ? kSmallTile : kBM);
+  split_plan(M, N, K, batch, &nsplit, &kchunk, planned_tile(M, N, batch));
   return nsplit > 1 ? (size_t)nsplit * batch * M * N * elem_bytes(prec) : 0;
+}
+
+// ---- tile autotuner ---------------------------------------------------------
+int gemm_record_begin(std::vector<GemmCall>* calls) {
+  t_record = calls;
+  return RENO_OK;
+}
+void gemm_record_end() { t_record = nullptr; }
+
+bool gemm_is_tuned(const GemmCall& g) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return g_tuned.count(tune_key(g)) != 0;
+}
+
+// Time every candidate (tile 64 or 32) x (split of K into 1, 2, 3, 4, 6, 8,
+// 12 or 16 chunks of at least 32, partials within the call's scratch) of
+// one GEMM call on `stream` with HIP events, `reps` launches each after one
+// untimed launch, and store the fastest.  The planner's own choice is kept
+// unless a candidate beats it by more than 2 %.  Synchronises the stream.
+int gemm_tune(const GemmCall& g, hipStream_t stream, int reps, reno_tile_cfg* out) {
+  TimingHookSuspend quiet;
+  TileChoice planned{planned_tile(g.M, g.N, g.batch), 1};
+  {
+    int ns = 1, chunk = 1;
+    split_plan(g.M, g.N, g.K, g.batch, &ns, &chunk, planned.tile);
+    const size_t cell = (size_t)g.batch * g.M * g.N * elem_bytes(g.prec);
+    planned.nsplit = (ns > 1 && g.part && g.part_bytes >= ns * cell) ? ns : 1;
+  }
+  std::vector<TileChoice> cand{planned};
+  const size_t cell = (size_t)g.batch * g.M * g.N * elem_bytes(g.prec);
+  for (int tile : {kBM, kSmallTile}) {
+    std::vector<int> seen;
+    for (int s : {1, 2, 3, 4, 6, 8, 12, 16}) {
+      int ns, chunk;
+      split_fixed(g.K, s, &ns, &chunk);
+      if (ns > 1 && (chunk < 32 || !g.part || g.part_bytes < ns * cell)) continue;
+      if (std::find(seen.begin(), seen.end(), ns) != seen.end()) continue;
+      seen.push_back(ns);
+      if (tile == planned.tile && ns == planned.nsplit) continue;
+      cand.push_back(TileChoice{tile, ns});
+    }
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  RENO_HIP_CHECK(hipEventCreate(&e0));
+  RENO_HIP_CHECK(hipEventCreate(&e1));
+  std::vector<float> us(cand.size(), 0.0f);
+  int st = RENO_OK;
+  for (size_t i = 0; i < cand.size() && st == RENO_OK; ++i) {
+    t_force = &cand[i];
+    st = launch_gemm(g, stream);
+    if (st == RENO_OK && hipEventRecord(e0, stream) != hipSuccess) st = RENO_EHIP;
+    for (int r = 0; r < reps && st == RENO_OK; ++r) st = launch_gemm(g, stream);
+    if (st == RENO_OK && hipEventRecord(e1, stream) != hipSuccess) st = RENO_EHIP;
+    if (st == RENO_OK && hipEventSynchronize(e1) != hipSuccess) st = RENO_EHIP;
+    float ms = 0.0f;
+    if (st == RENO_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = RENO_EHIP;
+    us[i] = 1e3f * ms / (float)(reps > 0 ? reps : 1);
+    t_force = nullptr;
+  }
+  t_force = nullptr;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != RENO_OK) {
+    if (st == RENO_EHIP) set_error("autotune: HIP event failure");
+    return st;
+  }
+  size_t best = 0;
+  for (size_t i = 1; i < cand.size(); ++i)
+    if (us[i] < us[best]) best = i;
+  if (best != 0 && us[best] > 0.98f * us[0]) best = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[tune_key(g)] = cand[best];
+    g_tuned_count.store((int)g_tuned.size(), std::memory_order_release);
+  }
+  if (out) {
+    out->M = g.M; out->N = g.N; out->K = g.K; out->batch = g.batch; out->prec = g.prec;
+    out->tile = cand[best].tile; out->nsplit = cand[best].nsplit;
+    out->planned_tile = planned.tile; out->planned_nsplit = planned.nsplit;
+    out->us_tuned = us[best]; out->us_planned = us[0];
+  }
+  return RENO_OK;
+}
+
+int gemm_tune_reset() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned.clear();
+  g_tuned_count.store(0, std::memory_order_release);
+  return RENO_OK;
+}
+
+int gemm_tune_entries() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return (int)g_tuned.size();
 }
 
 int prof_enable(int period) {

@@ -5,6 +5,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <vector>
+
 #include "../../include/reno_hip.h"
 
 namespace reno {
@@ -103,6 +105,15 @@ struct TimingHookSuspend {
 };
 // bytes of split-K scratch launch_gemm would use for this shape (0: no split)
 size_t splitk_bytes(int prec, int M, int N, int K, int batch);
+// tile autotuner (gemm.hip): record this thread's GEMM calls instead of
+// launching them (between begin and end); whether a call's signature is in
+// the tuned table; time the candidates of one call and store the fastest
+int gemm_record_begin(std::vector<GemmCall>* calls);
+void gemm_record_end();
+bool gemm_is_tuned(const GemmCall& g);
+int gemm_tune(const GemmCall& g, hipStream_t stream, int reps, reno_tile_cfg* out);
+int gemm_tune_reset();
+int gemm_tune_entries();
 
 inline size_t align_up(size_t x, size_t a = 256) { return (x + a - 1) / a * a; }
 

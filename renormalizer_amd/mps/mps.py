@@ -514,6 +514,8 @@ class Mps(MatrixProduct):
             environ = Environ(mps._mp, wt, "L" if not mps.to_right else "R", prec=prec)
         use_async = budgets is not None
         adaptive = use_async and tau is not None
+        # tile autotuning of the matvec GEMMs (once per operator shape)
+        tune = use_async and getattr(self.evolve_config, "autotune", False)
         if adaptive:
             wt32 = mpo.device_tensors(native.PREC_MIXED, dev)
         if use_async:
@@ -533,12 +535,16 @@ class Mps(MatrixProduct):
             prev = budgets.get(key, 0)
             budget = prev + (prev // 4 + 8 if wide else prev // 16 + 6)
             budget = max(budget, 12)
+            if tune:
+                native.autotune(l_array, w, r_array, v, prec)
             if adaptive:
                 js = switches.get(key, native.SWITCH_NEVER) if switches else native.SWITCH_NEVER
                 w32 = None if w is None else wt32[key[1]]
+                l32, r32 = l_array.to(torch.complex64), r_array.to(torch.complex64)
+                if tune:
+                    native.autotune(l32, w32, r32, v, native.PREC_MIXED)
                 out = native.expm_krylov_mixed_async(
-                    l_array, w, r_array, l_array.to(torch.complex64), w32, r_array.to(torch.complex64),
-                    v, dt, budget, js, tau, info[len(keys)])
+                    l_array, w, r_array, l32, w32, r32, v, dt, budget, js, tau, info[len(keys)])
             else:
                 out = native.expm_krylov_async(l_array, w, r_array, v, dt, prec, budget, info[len(keys)])
             keys.append(key)

@@ -35,6 +35,7 @@ EXPORTS = (
     "reno_hop_apply", "reno_hop_apply_noalloc", "reno_env_update", "reno_env_close", "reno_expm_krylov",
     "reno_expm_krylov_async", "reno_expm_krylov_mixed_async", "reno_block_qr", "reno_heff_diag", "reno_heff_dense", "reno_zgemm",
     "reno_batch_scratch_bytes", "reno_batch_site_step", "reno_batch_normalize",
+    "reno_autotune", "reno_autotune_reset", "reno_autotune_entries",
     "reno_probe_mfma16", "reno_prof_enable", "reno_prof_collect", "reno_prof_kind_info",
     "reno_prof_cancelled",
 )
@@ -55,6 +56,15 @@ def gemm_kind_names():
             "double" if info[0] == PREC_FP64 else "float", "true" if info[1] else "false",
             "true" if info[2] else "false", info[3], info[4], info[5]))
     return names
+
+
+class TileCfg(ctypes.Structure):
+    """reno_tile_cfg: one tuned GEMM stage."""
+    _fields_ = [("M", ctypes.c_int), ("N", ctypes.c_int), ("K", ctypes.c_int),
+                ("batch", ctypes.c_int), ("prec", ctypes.c_int), ("tile", ctypes.c_int),
+                ("nsplit", ctypes.c_int), ("planned_tile", ctypes.c_int),
+                ("planned_nsplit", ctypes.c_int), ("us_tuned", ctypes.c_float),
+                ("us_planned", ctypes.c_float)]
 
 
 class RenoNativeError(RuntimeError):
@@ -134,6 +144,13 @@ def load_library(path: str = LIB_PATH):
         lib.reno_batch_site_step.argtypes = [c_int, vp, c_double, vp]
         lib.reno_batch_normalize.restype = c_int
         lib.reno_batch_normalize.argtypes = [c_int, vp, vp, vp, vp]
+        lib.reno_autotune.restype = c_int
+        lib.reno_autotune.argtypes = [ctypes.POINTER(HopDesc), vp, vp, vp, c_size, c_int,
+                                      ctypes.POINTER(TileCfg), c_int, ctypes.POINTER(c_int), vp]
+        lib.reno_autotune_reset.restype = c_int
+        lib.reno_autotune_reset.argtypes = []
+        lib.reno_autotune_entries.restype = c_int
+        lib.reno_autotune_entries.argtypes = []
         lib.reno_probe_mfma16.restype = c_int
         lib.reno_probe_mfma16.argtypes = [c_int, vp, vp, vp, vp]
         lib.reno_prof_enable.restype = c_int
@@ -271,6 +288,58 @@ def hop_apply(L, W, R, c, prec=PREC_FP64):
     check(l.reno_hop_apply(ctypes.byref(desc), _ptr(c), _ptr(out), _ptr(ws), nbytes, _stream()),
           "reno_hop_apply")
     return out
+
+
+_TUNED = set()
+_TUNED_LOCK = threading.Lock()
+# every stage record the tuner returned in this process (bench reports them)
+tuned_records = []
+
+
+def autotune(L, W, R, c, prec=PREC_FP64, reps=8):
+    """Tune the GEMM stages of the matvec hop_apply(L, W, R, c, prec) on the
+    device (reno_autotune): every tile / split-K candidate is timed on the
+    current stream and the fastest is used by all later launches of the same
+    GEMM signatures.  Operators already tuned in this process return at once.
+    Returns the records of the stages tuned by this call."""
+    _require_device(L, R, c, W)
+    _check_env_dtype(prec, L, R, W)
+    if W is None:
+        Ml, Mr = c.shape
+        D = L.shape[1]
+        desc = _hop_desc(0, prec, L, None, R, Ml, 1, Mr, D, D)
+    else:
+        Ml, d, Mr = c.shape
+        desc = _hop_desc(1, prec, L, W, R, Ml, d, Mr, W.shape[0], W.shape[3])
+    key = (desc.nsite, prec, desc.Ml, desc.d, desc.Mr, desc.Dl, desc.Dr, c.device.index)
+    with _TUNED_LOCK:
+        if key in _TUNED:
+            return []
+    l = lib()
+    nbytes = l.reno_hop_workspace_bytes(ctypes.byref(desc))
+    ws = workspace.get(nbytes, c.device)
+    out = torch.empty_like(c)
+    recs = (TileCfg * 4)()
+    n = ctypes.c_int(0)
+    check(l.reno_autotune(ctypes.byref(desc), _ptr(c), _ptr(out), _ptr(ws), nbytes, int(reps),
+                          recs, 4, ctypes.byref(n), _stream()), "reno_autotune")
+    got = [{f: getattr(recs[i], f) for f, _ in TileCfg._fields_} for i in range(n.value)]
+    with _TUNED_LOCK:
+        _TUNED.add(key)
+        tuned_records.extend(got)
+    return got
+
+
+def autotune_reset():
+    """Forget every tuned choice (library table and this module's record)."""
+    check(lib().reno_autotune_reset(), "reno_autotune_reset")
+    with _TUNED_LOCK:
+        _TUNED.clear()
+        tuned_records.clear()
+
+
+def autotune_entries():
+    return int(lib().reno_autotune_entries())
 
 
 def env_update(env, ms, W, domain, ms_conj=None, prec=PREC_FP64):

@@ -9,6 +9,10 @@ weighs the Lanczos vectors it creates by at least ``mixed_tau``, on FP32
 matrix cores with complex64 copies of the environments after that);
 "fp32" (every contraction on FP32 matrix cores with complex64
 environments; kept for comparison, it misses the 1e-6 population bound).
+``EvolveConfig.autotune`` picks the GEMM tile and split-K of each matvec
+shape by timing the candidates on the device (off by default: the choice
+depends on measured times, so results can differ between runs at rounding
+level).
 """
 from enum import Enum
 from typing import Union
@@ -164,7 +168,7 @@ class EvolveConfig:
                  adaptive=False, guess_dt=1e-1, adaptive_rtol=5e-4, taylor_order: int = None,
                  rk_solver="C_RK4", reg_epsilon=1e-10, ivp_rtol=1e-5, ivp_atol=1e-8,
                  ivp_solver="krylov", force_ovlp=True, precision: str = "fp64",
-                 krylov_async: bool = True, mixed_tau: float = 1e-2):
+                 krylov_async: bool = True, mixed_tau: float = 1e-2, autotune: bool = False):
         if isinstance(method, str):
             method = getattr(EvolveMethod, method)
         self.method = method
@@ -194,6 +198,9 @@ class EvolveConfig:
         # coefficient of the previous Lanczos vector in the step's result is
         # below mixed_tau times the norm of the starting vector
         self.mixed_tau = float(mixed_tau)
+        # asynchronous device sweep: time the tile / split-K candidates of
+        # each new matvec shape once (native.autotune) and use the fastest
+        self.autotune = bool(autotune)
         self.stat = None
 
     @property

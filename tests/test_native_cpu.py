@@ -1,5 +1,6 @@
 """The C-ABI library builds, loads without a GPU, exports every symbol that
 include/reno_hip.h declares, and refuses host tensors (no CPU fallback)."""
+import ctypes
 import os
 import re
 
@@ -67,6 +68,32 @@ def test_invalid_arguments_rejected_without_launch():
     assert b"precision" in lib.reno_last_error()
     st = lib.reno_block_qr(0, 4, 4, 1, 0, None, 1, 1, 0, 1, 1 << 20, None)
     assert st == native.RENO_EQN
+
+
+def test_autotune_arguments_and_table_without_launch():
+    """reno_autotune refuses bad arguments before any device work; the tuned
+    table starts empty and reset leaves it empty."""
+    lib = native.load_library()
+    assert lib.reno_autotune_reset() == native.RENO_OK
+    assert lib.reno_autotune_entries() == 0
+    d = native.HopDesc(1, native.PREC_FP64, 4, 2, 4, 1, 1, 1, 1, 1)
+    n = ctypes.c_int(-1)
+    recs = (native.TileCfg * 4)()
+    # aliased vectors
+    st = lib.reno_autotune(d, 5, 5, 3, 1 << 20, 4, recs, 4, ctypes.byref(n), None)
+    assert st == native.RENO_EINVAL and n.value == 0
+    assert b"aliased" in lib.reno_last_error()
+    # repetitions out of range
+    st = lib.reno_autotune(d, 5, 6, 3, 1 << 20, 0, recs, 4, ctypes.byref(n), None)
+    assert st == native.RENO_EINVAL
+    # workspace too small
+    st = lib.reno_autotune(d, 5, 6, 3, 16, 4, recs, 4, ctypes.byref(n), None)
+    assert st == native.RENO_ENOSPACE
+    # unknown precision
+    bad = native.HopDesc(1, 7, 4, 2, 4, 1, 1, 1, 1, 1)
+    assert lib.reno_autotune(bad, 5, 6, 3, 1 << 20, 4, recs, 4, ctypes.byref(n), None) == native.RENO_EINVAL
+    assert lib.reno_autotune_entries() == 0
+    assert ctypes.sizeof(native.TileCfg) == 44
 
 
 def test_krylov_retry_terminates():
