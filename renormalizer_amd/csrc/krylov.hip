@@ -255,6 +255,18 @@ __global__ __launch_bounds__(64) void k_tri_coef(const double* __restrict__ alph
                                                  const double* __restrict__ scal, double dt_re,
                                                  double dt_im, int m_in, int final,
                                                  const int* st, double2* __restrict__ coef) {
+  // the operands first (alpha and beta hold at least m_in entries), so that
+  // their latency overlaps the reads of the state block
+  const int lane = threadIdx.x & 63;
+  double a[R], bl[R], br[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = lane * R + r;
+    a[r] = i < m_in ? alpha[i] : 0.0;
+    bl[r] = (i > 0 && i < m_in) ? beta[i - 1] : 0.0;
+    br[r] = i + 1 < m_in ? beta[i] : 0.0;
+  }
+  const double v0n = scal[0];
   if (!final && check_void(st, m_in - 1)) return;
   if (final) {
     const int kind = final_kind(st);
@@ -263,7 +275,13 @@ __global__ __launch_bounds__(64) void k_tri_coef(const double* __restrict__ alph
   __shared__ double sJ[kChebMax];
   __shared__ int sK;
   const int m = final ? st[KS_MFINAL] : m_in;
-  tri_coef_wave<R>(alpha, beta, scal[0], dt_re, dt_im, m, sJ, &sK, coef);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = lane * R + r;
+    if (i >= m) a[r] = bl[r] = 0.0;
+    if (i + 1 >= m) br[r] = 0.0;
+  }
+  tri_coef_core<R>(a, bl, br, v0n, dt_re, dt_im, m, sJ, &sK, coef);
 }
 
 // Convergence check of iteration j (krylov.py:76-81): with

@@ -33,29 +33,54 @@ __device__ __forceinline__ double wave_shl1(double v) {
 constexpr int kChebMax = 512;  // Bessel terms of the Chebyshev branch
 
 // J_k(x), k = 0 .. nmax - 1, for x > 0 by Miller's backward recurrence
-// J_{k-1} = (2k / x) J_k - J_{k+1} from a start well above nmax (one
-// thread).  Stores the unnormalised values in J[0..nmax) and the
-// normalisation sum J_0 + 2 sum_{k>=1} J_{2k} in J[nmax]; the caller's wave
-// divides by it (J holds nmax + 1 doubles).
+// J_{k-1} = (2k / x) J_k - J_{k+1} from a start well above nmax.  Every lane
+// of the calling wave runs the same recurrence, so there is no divergent
+// branch in the loop: lane l keeps J_{64 q + l} while block q of indices
+// goes by, and the wave stores each finished block of 64 at once.  Writes
+// the unnormalised values to J[0..nmax) and the normalisation sum
+// J_0 + 2 sum_{k>=1} J_{2k} to J[nmax] (J holds nmax + 1 doubles).  The
+// overflow test runs every 8 terms: 8 terms grow by less than
+// (2 * 512 / 0.05)^8 ~ 3e34, far inside the double range above its 1e200
+// threshold.
 __device__ inline void bessel_j_miller(double x, int nmax, double* J) {
-  const int start = nmax + 24 + (nmax & 1);
+  const int lane = threadIdx.x & 63;
+  const int start = nmax + 24 + (nmax & 1);  // even
   const double ix = 1.0 / x;
-  double jp1 = 0.0, jk = 1e-30, norm = 0.0;
-  for (int k = start; k >= 1; --k) {
-    const double jm1 = 2.0 * k * ix * jk - jp1;
-    if (k - 1 < nmax) J[k - 1] = jm1;
-    if (((k - 1) & 1) == 0 && k - 1 > 0) norm += 2.0 * jm1;
-    if (k - 1 == 0) norm += jm1;
+  double jp1 = 0.0, jk = 1e-30, norm = 0.0, cur = 0.0;
+  // one term: the value of index i (= k - 1), indices descending
+  auto step = [&](int i) {
+    const double c = (double)(2 * (i + 1)) * ix;
+    const double jm1 = c * jk - jp1;
+    cur = ((i & 63) == lane && i < nmax) ? jm1 : cur;
+    if ((i & 1) == 0) norm = i > 0 ? norm + 2.0 * jm1 : norm + jm1;
     jp1 = jk;
     jk = jm1;
+  };
+  // after the term of index i, i a multiple of 8
+  auto block_end = [&](int i) {
     if (fabs(jk) > 1e200) {  // rescale everything computed so far
       jp1 *= 1e-200;
       jk *= 1e-200;
       norm *= 1e-200;
-      for (int i = k - 1; i < nmax; ++i) J[i] *= 1e-200;
+      cur *= 1e-200;
+      for (int t = ((i >> 6) + 1) * 64 + lane; t < nmax; t += 64) J[t] *= 1e-200;
     }
+    if ((i & 63) == 0) {
+      if (i + lane < nmax) J[i + lane] = cur;
+      cur = 0.0;
+    }
+  };
+  int i = start - 1;
+  const int top = i & 7;
+  for (int t = 0; t <= top; ++t, --i) step(i);
+  block_end(i + 1);
+  while (i >= 0) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) step(i - t);
+    i -= 8;
+    block_end(i + 1);
   }
-  J[nmax] = norm;
+  if (lane == 0) J[nmax] = norm;
 }
 
 __device__ __forceinline__ int wave_max_int(int v) {
@@ -79,6 +104,14 @@ __device__ __forceinline__ int wave_max_int(int v) {
 // operation order.  Every thread of the workgroup must call it (it holds
 // workgroup barriers); sJ (kChebMax doubles) and sK are the calling wave's
 // LDS, coef (m entries) is written unless null.
+// tri_coef_core: the same with this lane's entries of T already in
+// registers (a = alpha, bl / br = the off-diagonal to the left / right,
+// zero past m), for callers that load them early.
+template <int R>
+__device__ __forceinline__ void tri_coef_core(double (&a)[R], double (&bl)[R], double (&br)[R],
+                                              double v0n, double dt_re, double dt_im, int m,
+                                              double* sJ, int* sK, double2* __restrict__ coef);
+
 template <int R>
 __device__ __forceinline__ void tri_coef_wave(const double* __restrict__ alpha,
                                               const double* __restrict__ beta, double v0n,
@@ -86,7 +119,6 @@ __device__ __forceinline__ void tri_coef_wave(const double* __restrict__ alpha,
                                               int* sK, double2* __restrict__ coef) {
   const int lane = threadIdx.x & 63;
   double a[R], bl[R], br[R];
-  double lo = 1e300, hi = -1e300;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int i = lane * R + r;
@@ -94,6 +126,20 @@ __device__ __forceinline__ void tri_coef_wave(const double* __restrict__ alpha,
     a[r] = in ? alpha[i] : 0.0;
     bl[r] = (in && i > 0) ? beta[i - 1] : 0.0;
     br[r] = (in && i + 1 < m) ? beta[i] : 0.0;
+  }
+  tri_coef_core<R>(a, bl, br, v0n, dt_re, dt_im, m, sJ, sK, coef);
+}
+
+template <int R>
+__device__ __forceinline__ void tri_coef_core(double (&a)[R], double (&bl)[R], double (&br)[R],
+                                              double v0n, double dt_re, double dt_im, int m,
+                                              double* sJ, int* sK, double2* __restrict__ coef) {
+  const int lane = threadIdx.x & 63;
+  double lo = 1e300, hi = -1e300;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = lane * R + r;
+    const bool in = i < m;
     if (in) {
       lo = fmin(lo, a[r] - fabs(bl[r]) - fabs(br[r]));
       hi = fmax(hi, a[r] + fabs(bl[r]) + fabs(br[r]));
@@ -109,7 +155,7 @@ __device__ __forceinline__ void tri_coef_wave(const double* __restrict__ alpha,
   const double x = fabs(w);
   const int nterm = (int)(x + 12.0 * cbrt(x) + 32.0);
   if (dt_re == 0.0 && x >= 0.05 && nterm < kChebMax - 24) {
-    if (lane == 0) bessel_j_miller(x, nterm, sJ);
+    bessel_j_miller(x, nterm, sJ);
     __syncthreads();
     // normalise across the wave; K = number of leading terms to keep (past
     // the last one every |J_k| < 1e-18)
@@ -143,7 +189,8 @@ __device__ __forceinline__ void tri_coef_wave(const double* __restrict__ alpha,
       accr[r] = sJ[0] * yp[r];
       acci[r] = 0.0;
     }
-    for (int k = 1; k < K; ++k) {
+    // one Chebyshev term k with phase index ph = k mod 4 and J_k = jk
+    auto term = [&](int k, int ph, double jk) {
       const double lr = wave_shr1(yp[R - 1]);
       const double rr = wave_shl1(yp[0]);
       double pv = lr;
@@ -154,8 +201,7 @@ __device__ __forceinline__ void tri_coef_wave(const double* __restrict__ alpha,
         t[r] = a[r] * yp[r] + bl[r] * pv + br[r] * nv;  // Y T_{k-1}(Y) e_0
         pv = yp[r];
       }
-      const double ck = 2.0 * sJ[k];
-      const int ph = k & 3;
+      const double ck = 2.0 * jk;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         // T_k = 2 Y T_{k-1} - T_{k-2}  (T_1 = Y T_0)
@@ -168,7 +214,19 @@ __device__ __forceinline__ void tri_coef_wave(const double* __restrict__ alpha,
         else if (ph == 2) accr[r] -= v;
         else acci[r] -= s * v;
       }
+    };
+    // k = 1, then blocks of four from k = 2 (phases 2, 3, 0, 1 fixed at
+    // compile time, the four J_k loaded together), then the rest
+    int k = 1;
+    if (K > 1) term(1, 1, sJ[1]);
+    for (k = 2; k + 3 < K; k += 4) {
+      const double j0 = sJ[k], j1 = sJ[k + 1], j2 = sJ[k + 2], j3 = sJ[k + 3];
+      term(k, 2, j0);
+      term(k + 1, 3, j1);
+      term(k + 2, 0, j2);
+      term(k + 3, 1, j3);
     }
+    for (; k < K; ++k) term(k, k & 3, sJ[k]);
     const double pr = v0n * cos(dt_im * c), pi = v0n * sin(dt_im * c);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
