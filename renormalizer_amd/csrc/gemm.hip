@@ -139,14 +139,18 @@ __device__ __forceinline__ void cgemm_body(const GemmCall& g) {
   const int wn = wave & 1;
 
   // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
-  // give each XCD a contiguous run of tiles (shared A row panels in its L2)
+  // give each XCD a contiguous run of tiles.  Row-major runs share A row
+  // panels and read all of B on every XCD; column-major runs share B column
+  // panels and read all of A: the launcher picks the order that replicates
+  // the smaller operand across the XCDs' L2 caches.
   const int tilesN = (g.N + kBN - 1) / kBN;
+  const int tilesM = (g.M + kBM - 1) / kBM;
   const int ntiles = gridDim.x;
   const int t8 = (ntiles / 8) * 8;
   const int bid = blockIdx.x;
   const int tile = bid < t8 ? (bid % 8) * (ntiles / 8) + bid / 8 : bid;
-  const int tm = tile / tilesN;
-  const int tn = tile - tm * tilesN;
+  const int tm = g.tile_colmajor ? tile % tilesM : tile / tilesN;
+  const int tn = g.tile_colmajor ? tile / tilesM : tile - (tile / tilesN) * tilesN;
   const int m0 = tm * kBM;
   const int n0 = tn * kBN;
   const int64_t bat = blockIdx.y;
@@ -705,6 +709,8 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
   const bool bk = (g.sBk == 1);
   GemmCall k = g;
   k.kchunk = kchunk;
+  // same K for both operands: M < N means A is the smaller one
+  k.tile_colmajor = g.M < g.N;
   int kid = -1;
   if (nsplit > 1) {
     // partial sums [split][batch][M][N] in the compute precision, overwritten
@@ -719,6 +725,7 @@ int launch_gemm(const GemmCall& g, hipStream_t stream) {
       nsplit = 1;
       k = g;
       k.kchunk = g.K > 0 ? g.K : 1;
+      k.tile_colmajor = g.M < g.N;
     }
   }
   if (kid < 0) kid = find_kind(k, ak, bk);

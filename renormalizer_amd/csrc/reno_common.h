@@ -93,6 +93,10 @@ struct GemmCall {
   const double2* dot_v;
   double* dot_part;
   int dot_np;
+  // set by launch_gemm: tiles numbered column-major (each XCD's run of
+  // tiles then covers a block of columns and reads all of A and its own
+  // part of B) rather than row-major
+  int tile_colmajor;
 };
 
 int launch_gemm(const GemmCall& g, hipStream_t stream);
