@@ -43,7 +43,11 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_workload(nmode=64, nlev=10, m=256):
+def build_workload(nmode=64, nlev=10, m=256, cache=None):
+    """The C4 model, its MPO and the expanded initial state.  With ``cache``
+    (a .npz path) the initial state is loaded from there when the file
+    exists (Mps.load: the same tensors and quantum numbers bit for bit) and
+    written there after the expansion otherwise."""
     from renormalizer_amd import (CompressConfig, CompressCriteria, EvolveConfig, EvolveMethod, Mpo, Mps,
                                   Phonon, Quantity, SpinBosonModel)
     from renormalizer_amd.sbm import SpectralDensityFunction
@@ -53,10 +57,17 @@ def build_workload(nmode=64, nlev=10, m=256):
     model = SpinBosonModel(Quantity(0.0), Quantity(1.0),
                            [Phonon.simple_phonon(o, d, nlev) for o, d in zip(omegas, dis)])
     mpo = Mpo(model)
-    mps = Mps.ground_state(model, False)
+    if cache and os.path.exists(cache):
+        mps = Mps.load(model, cache)
+    else:
+        mps = Mps.ground_state(model, False)
+        mps.compress_config = CompressConfig(CompressCriteria.fixed, max_bonddim=m)
+        mps.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps)
+        mps = mps.expand_bond_dimension(mpo, coef=1e-16, include_ex=False)
+        if cache:
+            mps.dump(cache)
     mps.compress_config = CompressConfig(CompressCriteria.fixed, max_bonddim=m)
     mps.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps)
-    mps = mps.expand_bond_dimension(mpo, coef=1e-16, include_ex=False)
     return model, mpo, mps
 
 
@@ -417,6 +428,8 @@ def run_c3(instances, steps, warmup, seed=2026, dt=0.02):
     t_single = time.perf_counter() - t0
     del ref
     be.step()
+    # every later step: one replay of a HIP graph holding its launches
+    be.capture()
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -442,7 +455,7 @@ def run_c3(instances, steps, warmup, seed=2026, dt=0.02):
         "config": {"workload": f"C3: {instances} Holstein chains of 8..32 molecules (scheme 2, 8 levels, "
                                f"omega 1, g 0.7, J -1, site energies N(0, 0.5), seed {seed}), "
                                f"M = 16/32/64, dt {dt}, TDVP-PS one-site, FP64",
-                   "launches_per_step": launches},
+                   "launches_per_step": launches, "hip_graph": True},
         "one_state_path_4_streams_ms_per_step": round(1e3 * t_single, 1),
         "speedup_vs_one_state_path": round(t_single / t_grouped, 2),
         "roofline": {"bound": "mfma", "achieved": round(flop / steps / (gpu_ms * 1e-3) / 1e12, 3),
@@ -481,6 +494,36 @@ def _rocprof_csv(directory, suffix):
         return list(csv.DictReader(f))
 
 
+def traffic_probe(reps=20):
+    """The matvec stages at the C4 full-site shape (M = 256, d = 10, D = 3)
+    in both precisions, tuned as the sweep tunes them, each matvec followed
+    by a device synchronisation (counter runs: rocprofv3 --pmc)."""
+    import torch
+    from renormalizer_amd import native
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    gen = torch.Generator().manual_seed(0)
+    M, d, D = 256, 10, 3
+
+    def cplx(*shape):
+        return torch.complex(torch.randn(*shape, generator=gen, dtype=torch.float64),
+                             torch.randn(*shape, generator=gen, dtype=torch.float64))
+    for prec in (native.PREC_FP64, native.PREC_MIXED):
+        cd = torch.complex128 if prec == native.PREC_FP64 else torch.complex64
+        rd = torch.float64 if prec == native.PREC_FP64 else torch.float32
+        L, R = cplx(M, D, M).to(cd).to(dev), cplx(M, D, M).to(cd).to(dev)
+        W = torch.randn(D, d, d, D, generator=gen, dtype=torch.float64).to(rd).to(dev)
+        c1, c0 = cplx(M, d, M).to(dev), cplx(M, M).to(dev)
+        native.autotune(L, W, R, c1, prec)
+        native.autotune(L, None, R, c0, prec)
+        for _ in range(reps):
+            native.hop_apply(L, W, R, c1, prec)
+            torch.cuda.synchronize()
+            native.hop_apply(L, None, R, c0, prec)
+            torch.cuda.synchronize()
+    log(f"traffic probe: {reps} one-site and bond matvecs per precision at M = {M}, d = {d}, D = {D}")
+
+
 def summarize_counters(directory, top=12):
     """Per-kernel table of a rocprofv3 --pmc run: dispatches, dispatches
     whose counter reads under 1 KiB (launches cancelled on the device by the
@@ -488,9 +531,11 @@ def summarize_counters(directory, top=12):
     import collections
     import math
     agg = collections.defaultdict(list)
+    grid = collections.defaultdict(list)
     counter = None
     for r in _rocprof_csv(directory, "counter_collection.csv"):
         agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        grid[(r["Kernel_Name"], r.get("Grid_Size", "?"))].append(float(r["Counter_Value"]))
         counter = r.get("Counter_Name", counter)
     print(f"counter {counter}")
     print("| kernel | dispatches | under 1 KiB | sum KiB | log2(KiB) histogram |")
@@ -499,6 +544,11 @@ def summarize_counters(directory, top=12):
         hist = collections.Counter(int(math.log2(x)) if x >= 1 else -1 for x in v)
         h = " ".join(f"{k}:{hist[k]}" for k in sorted(hist))
         print(f"| {name} | {len(v)} | {sum(1 for x in v if x < 1)} | {sum(v):.1f} | {h} |")
+    print("\nby grid size (work-items):\n")
+    print("| kernel | grid | dispatches | mean KiB | min KiB | max KiB |")
+    print("|---|---|---|---|---|---|")
+    for (name, gs), v in sorted(grid.items(), key=lambda kv: -sum(kv[1]))[:3 * top]:
+        print(f"| {name} | {gs} | {len(v)} | {sum(v) / len(v):.1f} | {min(v):.1f} | {max(v):.1f} |")
 
 
 def summarize_trace(directory, top=12):
@@ -528,6 +578,9 @@ def main():
     ap.add_argument("--mixed-tau", type=float, default=None,
                     help="switch threshold of the mixed mode (EvolveConfig.mixed_tau)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--state-cache", metavar="NPZ", default=None,
+                    help="load the initial state from this file if it exists, else build and "
+                         "write it there (profiling runs)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="time the tile / split-K candidates of each matvec shape during the "
                          "warm-up (EvolveConfig.autotune) and use the fastest")
@@ -540,6 +593,8 @@ def main():
                     help="time the numpy oracle alone on one workload (no GPU)")
     ap.add_argument("--cpu-sweep", action="store_true",
                     help="cpu-trajectory of C1/C2 at 1, 4 and all threads and C4 at all threads")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run only the C4-shape matvec stages, synchronously (PMC counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 512 MiB device copy (PMC counter calibration)")
     ap.add_argument("--summarize-counters", metavar="DIR",
@@ -555,6 +610,9 @@ def main():
         return
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.traffic_probe:
+        traffic_probe()
         return
     if a.cpu_trajectory:
         cpu_trajectory(a.cpu_trajectory, a.steps)
@@ -576,7 +634,7 @@ def main():
     import torch
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
-    model, mpo, mps0 = build_workload(a.modes, 10, a.bond_dim)
+    model, mpo, mps0 = build_workload(a.modes, 10, a.bond_dim, cache=a.state_cache)
     log(f"initial state (host expansion) {time.perf_counter() - t0:.1f} s, bond dims max "
         f"{max(mps0.bond_dims)}, sites {len(mps0)}")
 
