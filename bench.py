@@ -551,6 +551,39 @@ def summarize_counters(directory, top=12):
         print(f"| {name} | {gs} | {len(v)} | {sum(v) / len(v):.1f} | {min(v):.1f} | {max(v):.1f} |")
 
 
+def summarize_gaps(directory, last_ms=None):
+    """Busy and idle time of the device over a kernel trace: the union of
+    the kernel intervals against the span from the first start to the last
+    end (restricted to the last ``last_ms`` milliseconds of the trace when
+    given, e.g. the timed steps), and the histogram of the gaps between
+    consecutive kernels."""
+    import collections
+    iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
+                for r in _rocprof_csv(directory, "kernel_trace.csv"))
+    if last_ms is not None:
+        t_end = max(e for _, e in iv)
+        iv = [(s, e) for s, e in iv if s >= t_end - int(last_ms * 1e6)]
+    busy, cur_s, cur_e = 0, None, None
+    gaps = []
+    for s, e in iv:
+        if cur_e is None:
+            cur_s, cur_e = s, e
+        elif s <= cur_e:
+            cur_e = max(cur_e, e)
+        else:
+            busy += cur_e - cur_s
+            gaps.append(s - cur_e)
+            cur_s, cur_e = s, e
+    busy += cur_e - cur_s
+    span = max(e for _, e in iv) - iv[0][0]
+    hist = collections.Counter(min(int(g // 1000), 20) for g in gaps)
+    print(f"kernels {len(iv)}, span {span / 1e6:.1f} ms, busy {busy / 1e6:.1f} ms, "
+          f"idle {(span - busy) / 1e6:.1f} ms ({100 * (span - busy) / span:.1f} %), gaps {len(gaps)}, "
+          f"mean gap {sum(gaps) / max(len(gaps), 1) / 1e3:.2f} us")
+    print("gap histogram (us: count): " + ", ".join(
+        f"{k}{'+' if k == 20 else ''}: {hist[k]}" for k in sorted(hist)))
+
+
 def summarize_trace(directory, top=12):
     """Per-kernel table of a rocprofv3 --kernel-trace run: dispatches, total
     and average duration, and the number of dispatches shorter than 2 us
@@ -599,11 +632,18 @@ def main():
                     help="run only a plain 512 MiB device copy (PMC counter calibration)")
     ap.add_argument("--summarize-counters", metavar="DIR",
                     help="print the per-kernel table of a rocprofv3 --pmc output directory")
+    ap.add_argument("--summarize-gaps", metavar="DIR",
+                    help="device busy / idle time and kernel gaps of a rocprofv3 --kernel-trace directory")
+    ap.add_argument("--gaps-last-ms", type=float, default=None,
+                    help="with --summarize-gaps: only the last this many milliseconds of the trace")
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="print the per-kernel table of a rocprofv3 --kernel-trace output directory")
     a = ap.parse_args()
     if a.summarize_counters:
         summarize_counters(a.summarize_counters)
+        return
+    if a.summarize_gaps:
+        summarize_gaps(a.summarize_gaps, a.gaps_last_ms)
         return
     if a.summarize_trace:
         summarize_trace(a.summarize_trace)
