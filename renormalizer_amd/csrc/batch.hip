@@ -22,18 +22,20 @@
 
 namespace reno {
 
-constexpr int kBT = 256;      // threads per workgroup
+constexpr int kBT = 512;      // threads per workgroup: eight waves, two per SIMD
+constexpr int kWaves = kBT / 64;
 constexpr int kBTile = 64;    // GEMM output tile (four waves of 32 x 32)
+constexpr int kEngT = 256;    // threads of one GEMM tile engine (two per workgroup)
 constexpr int kBK = 16;       // GEMM K step
 
 typedef double d4b_t __attribute__((ext_vector_type(4)));
 
 struct BatchLds {
-  double ta[2][kBK][kBTile + 1];  // A tile, re / im planes, [k][m]
-  double tb[2][kBK][kBTile + 1];  // B tile, re / im planes, [k][n]
+  double ta[2][2][kBK][kBTile + 1];  // [engine] A tile, re / im planes, [k][m]
+  double tb[2][2][kBK][kBTile + 1];  // [engine] B tile, re / im planes, [k][n]
   double red[kBT];
-  double J[4][kChebMax];           // Bessel terms, one set per wave
-  int K[4];
+  double J[kWaves][kChebMax];         // Bessel terms, one set per wave
+  int K[kWaves];
 };
 
 // fixed-order workgroup sum, visible to every thread on return
@@ -57,16 +59,19 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // C[b] (+)= op(A[b]) op(B[b]) for one GemmCall (the strided batched product
 // of reno_common.h; A complex128 or float64, B and C complex128), computed by
-// the calling workgroup: 64 x 64 output tiles in turn, K in steps of 16
-// staged through LDS with the next step's elements already loaded into
-// registers while the current one is multiplied; each wave owns a 32 x 32
+// the calling workgroup: two 64 x 64 output tiles at a time (two engines of
+// four waves, in step), K in steps of 16 staged through LDS with the next
+// step's elements already loaded into registers while the current one is
+// multiplied; each wave of an engine owns a 32 x 32
 // quadrant of 2 x 2 v_mfma_f64_16x16x4f64 blocks (three real products per
 // complex product, two when A is real).  MFMA 16x16x4 f64 lane layout:
 // A lane l = A[l % 16][l / 16], B lane l = B[l / 16][l % 16], D lane l
 // register r = D[4 r + l / 16][l % 16].
 __device__ void wg_gemm(const GemmCall& g, BatchLds& s) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  // two tile engines of four waves each work on consecutive tiles in step
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int eng = wave >> 2, tid = threadIdx.x & (kEngT - 1);
+  const int wm = (wave & 3) >> 1, wn = wave & 1;
   const bool areal = g.a_kind == ST_F64;
   const int tm = (g.M + kBTile - 1) / kBTile, tn = (g.N + kBTile - 1) / kBTile;
   const double2* A2 = (const double2*)g.A;
@@ -74,10 +79,14 @@ __device__ void wg_gemm(const GemmCall& g, BatchLds& s) {
   const double2* B = (const double2*)g.B;
   double2* C = (double2*)g.C;
   const double sa = g.conjA ? -1.0 : 1.0, sb = g.conjB ? -1.0 : 1.0;
-  constexpr int kPer = kBTile * kBK / kBT;  // elements of each operand per thread
+  constexpr int kPer = kBTile * kBK / kEngT;  // elements of each operand per thread
   __syncthreads();  // the operands may have been written by this workgroup
   const int64_t ntile = (int64_t)g.batch * tm * tn;
-  for (int64_t bt = 0; bt < ntile; ++bt) {
+  for (int64_t bt0 = 0; bt0 < ntile; bt0 += 2) {
+    // the second engine idles (zero operands, no store) past the last tile
+    // but keeps to the workgroup's barriers
+    const bool valid = bt0 + eng < ntile;
+    const int64_t bt = valid ? bt0 + eng : bt0;
     const int64_t b = bt / (tm * tn);
     const int t = (int)(bt - b * tm * tn);
     const int m0 = (t / tn) * kBTile, n0 = (t % tn) * kBTile;
@@ -85,10 +94,10 @@ __device__ void wg_gemm(const GemmCall& g, BatchLds& s) {
     auto fetch = [&](int k0) {
 #pragma unroll
       for (int i = 0; i < kPer; ++i) {
-        const int e = tid + kBT * i;
+        const int e = tid + kEngT * i;
         const int x = e % kBTile, kk = e / kBTile;
         double re = 0.0, im = 0.0;
-        if (m0 + x < g.M && k0 + kk < g.K) {
+        if (valid && m0 + x < g.M && k0 + kk < g.K) {
           const int64_t off = b * g.sAb + (int64_t)(m0 + x) * g.sAm + (int64_t)(k0 + kk) * g.sAk;
           if (areal) {
             re = A1[off];
@@ -102,7 +111,7 @@ __device__ void wg_gemm(const GemmCall& g, BatchLds& s) {
         pa_i[i] = im;
         re = 0.0;
         im = 0.0;
-        if (n0 + x < g.N && k0 + kk < g.K) {
+        if (valid && n0 + x < g.N && k0 + kk < g.K) {
           const double2 v = B[b * g.sBb + (int64_t)(k0 + kk) * g.sBk + (int64_t)(n0 + x) * g.sBn];
           re = v.x;
           im = sb * v.y;
@@ -114,12 +123,12 @@ __device__ void wg_gemm(const GemmCall& g, BatchLds& s) {
     auto stash = [&]() {
 #pragma unroll
       for (int i = 0; i < kPer; ++i) {
-        const int e = tid + kBT * i;
+        const int e = tid + kEngT * i;
         const int x = e % kBTile, kk = e / kBTile;
-        s.ta[0][kk][x] = pa_r[i];
-        s.ta[1][kk][x] = pa_i[i];
-        s.tb[0][kk][x] = pb_r[i];
-        s.tb[1][kk][x] = pb_i[i];
+        s.ta[eng][0][kk][x] = pa_r[i];
+        s.ta[eng][1][kk][x] = pa_i[i];
+        s.tb[eng][0][kk][x] = pb_r[i];
+        s.tb[eng][1][kk][x] = pb_i[i];
       }
     };
     d4b_t cr[2][2], ci[2][2], cq[2][2];
@@ -139,10 +148,10 @@ __device__ void wg_gemm(const GemmCall& g, BatchLds& s) {
         double ar[2], ai[2], br[2], bi[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-          ar[q] = s.ta[0][kr][wm * 32 + q * 16 + (lane & 15)];
-          ai[q] = s.ta[1][kr][wm * 32 + q * 16 + (lane & 15)];
-          br[q] = s.tb[0][kr][wn * 32 + q * 16 + (lane & 15)];
-          bi[q] = s.tb[1][kr][wn * 32 + q * 16 + (lane & 15)];
+          ar[q] = s.ta[eng][0][kr][wm * 32 + q * 16 + (lane & 15)];
+          ai[q] = s.ta[eng][1][kr][wm * 32 + q * 16 + (lane & 15)];
+          br[q] = s.tb[eng][0][kr][wn * 32 + q * 16 + (lane & 15)];
+          bi[q] = s.tb[eng][1][kr][wn * 32 + q * 16 + (lane & 15)];
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
@@ -172,7 +181,7 @@ __device__ void wg_gemm(const GemmCall& g, BatchLds& s) {
         for (int r = 0; r < 4; ++r) {
           const int row = m0 + wm * 32 + i * 16 + 4 * r + (lane >> 4);
           const int col = n0 + wn * 32 + j * 16 + (lane & 15);
-          if (row < g.M && col < g.N) {
+          if (valid && row < g.M && col < g.N) {
             const double re = areal ? cr[i][j][r] : cr[i][j][r] - cq[i][j][r];
             const double im = areal ? ci[i][j][r] : ci[i][j][r] - cr[i][j][r] - cq[i][j][r];
             double2* p = C + b * g.sCb + (int64_t)row * g.sCm + col;
@@ -395,7 +404,7 @@ __device__ void wg_householder(double2* X, int m, int n, double2* Q, double2* ta
     __syncthreads();
     if (!(t.x == 0.0 && t.y == 0.0)) {
       const double2 ct = make_double2(t.x, -t.y);
-      for (int c = j + 1 + wave; c < n; c += 4) {
+      for (int c = j + 1 + wave; c < n; c += kWaves) {
         double2* xc = X + (int64_t)c * m;
         double dr = 0.0, di = 0.0;
         for (int r = j + 1 + lane; r < m; r += 64) {
@@ -426,7 +435,7 @@ __device__ void wg_householder(double2* X, int m, int n, double2* Q, double2* ta
     const double2 t = tau[i];
     const double2* xi = X + (int64_t)i * m;
     // H_i on columns c > i of Q: q_c -= tau v (v^H q_c), v = [1; X[i+1:, i]]
-    for (int c = i + 1 + wave; c < k; c += 4) {
+    for (int c = i + 1 + wave; c < k; c += kWaves) {
       double2* qc = Q + (int64_t)c * m;
       double dr = 0.0, di = 0.0;
       for (int r = i + 1 + lane; r < m; r += 64) {
