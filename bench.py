@@ -131,6 +131,47 @@ def run_mode(mps0, mpo, precision, dt, warmup, steps, profile, tau=None, autotun
             "matvecs_per_step": matvecs, "fp32_matvecs_per_step": low, "final": mps, "environ": env, "redone": redone}
 
 
+def torch_baseline(mps0, mpo, dt, steps, ref_sz):
+    """The same time steps by the plain-torch FP64 path (the reference's GPU
+    algorithm in torch operations: tensordot on the device, rocSOLVER QR,
+    Lanczos scalars and the tridiagonal eigenproblem on the host, every
+    environment rebuilt per step as the reference does) from the same
+    initial state: one untimed step, then ``steps`` timed ones; <sigma_z>
+    after each is compared with the HIP FP64 run's."""
+    import numpy as np
+    import torch
+    from renormalizer_amd.mps.torch_sweep import TorchSweepState
+    dev = torch.device("cuda", 0)
+    ts = TorchSweepState.from_mps(mps0, dev)
+    mpo_t = [torch.from_numpy(np.asarray(w)).to(device=dev, dtype=torch.complex128) for w in mpo]
+
+    def sz():
+        r = ts.calc_1site_rdm(0)
+        return float((r[0, 0] - r[1, 1]).real)
+
+    out_sz = [sz()]
+    ts.evolve(mpo_t, dt)
+    out_sz.append(sz())
+    seconds = 0.0
+    matvecs = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        counts = ts.evolve(mpo_t, dt)
+        torch.cuda.synchronize()
+        seconds += time.perf_counter() - t0
+        matvecs.append(sum(counts))
+        out_sz.append(sz())
+    n = min(len(out_sz), len(ref_sz))
+    diff = float(np.max(np.abs(np.array(out_sz[:n]) - np.array(ref_sz[:n]))))
+    log(f"torch fp64 baseline: {steps} timed steps in {seconds:.2f} s, matvecs/step {matvecs}")
+    return {"sweeps_per_s": round(steps / seconds, 5), "ms_per_step": round(1e3 * seconds / steps, 1),
+            "steps": steps, "warmup": 1, "matvecs_per_step": matvecs,
+            "max_abs_sigma_z_diff_vs_hip_fp64": diff,
+            "what": "plain torch FP64 (rocBLAS zgemm via tensordot, rocSOLVER QR, host Lanczos "
+                    "scalars; the reference's CuPy path restated in torch) on the same GPU and start"}
+
+
 TRAFFIC_PROFILES = [os.path.join(ROOT, "profiles", f"tdvp_c4_m256_{mode}_hbm_traffic.md")
                     for mode in ("fp64", "mixed")]
 
@@ -611,6 +652,8 @@ def main():
     ap.add_argument("--mixed-tau", type=float, default=None,
                     help="switch threshold of the mixed mode (EvolveConfig.mixed_tau)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--torch-steps", type=int, default=2,
+                    help="timed steps of the plain-torch FP64 baseline (0: skip it)")
     ap.add_argument("--state-cache", metavar="NPZ", default=None,
                     help="load the initial state from this file if it exists, else build and "
                          "write it there (profiling runs)")
@@ -741,6 +784,10 @@ def main():
         out["mixed_fp32_matvec_share"] = round(sum(res["mixed"]["fp32_matvecs_per_step"]) / mvmx, 3)
         other = "mixed" if main_mode == "fp64" else "fp64"
         out[other + "_roofline"] = roofline(res[other]["prof"], res[other]["seconds"], a.steps)
+    if a.torch_steps > 0 and "fp64" in res:
+        tb = torch_baseline(mps0, mpo, a.dt, a.torch_steps, res["fp64"]["sz"])
+        out["torch_fp64_baseline"] = tb
+        out["speedup_vs_torch_fp64"] = round(out["value"] / tb["sweeps_per_s"], 2)
     if not a.no_cpu:
         ref = res.get("fp64", r)
         log("cpu baseline sample")
