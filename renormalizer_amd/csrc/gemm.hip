@@ -466,17 +466,6 @@ __global__ void k_split_reduce(const typename std::conditional<sizeof(R) == 8, d
 }
 
 // 256-thread fixed-order sum (deterministic)
-__device__ __forceinline__ double block_sum256(double v, double* sh) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (tid < off) sh[tid] += sh[tid + off];
-    __syncthreads();
-  }
-  return sh[0];
-}
-
 // Split-K reduction of a contiguous complex128 result fused with the dot
 // product the Lanczos recurrence needs next: C = sum_s P[s] (+ C), and
 // dot_part[block] = sum over the block's share of Re(conj(C) v).

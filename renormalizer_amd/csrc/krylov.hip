@@ -85,14 +85,8 @@ static int n_partials(int64_t n) {
 }
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int off = kRedThreads / 2; off > 0; off >>= 1) {
-    if (tid < off) sh[tid] += sh[tid + off];
-    __syncthreads();
-  }
-  return sh[0];
+  static_assert(kRedThreads == 256, "block_sum256 sums 256 threads");
+  return block_sum256(v, sh);
 }
 
 // partial[b] = sum |v|^2
@@ -121,7 +115,6 @@ __device__ __forceinline__ bool fold_beta(const double* __restrict__ norm_part, 
   double b2 = 0.0;
   for (int i = threadIdx.x; i < np; i += kRedThreads) b2 += norm_part[i];
   const double be = sqrt(block_sum(b2, sh));
-  __syncthreads();
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     beta[j - 1] = be;
     sig[j] = 1.0 / be;
@@ -159,7 +152,6 @@ __global__ __launch_bounds__(kRedThreads) void k_recur(
   for (int i = threadIdx.x; i < np; i += kRedThreads) a += dot_part[i];
   const double al = block_sum(a, sh) * sj * sj;
   if (blockIdx.x == 0 && threadIdx.x == 0) alpha[j] = al;
-  __syncthreads();
   const double ca = al * sj;
   // beta_{j-1} was folded above (block 0 stores it concurrently, so it is
   // not read back); sig_{j-1} was stored by an earlier launch

@@ -121,6 +121,22 @@ int gemm_tune_entries();
 
 inline size_t align_up(size_t x, size_t a = 256) { return (x + a - 1) / a * a; }
 
+// Sum of v over the 256 threads of a block in a fixed order (every call
+// with the same inputs gives the same bits, and every thread returns the
+// same value): a butterfly over the 64 lanes of each wave (partners
+// lane ^ 32, ..., lane ^ 1; both partners add the same two operands), then
+// the four wave sums through sh[0..4).  Two barriers: the second lets the
+// caller reuse sh at once.
+__device__ __forceinline__ double block_sum256(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double s = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return s;
+}
+
 inline size_t elem_bytes(int prec) { return prec == RENO_PREC_FP64 ? 16 : 8; }
 
 }  // namespace reno
