@@ -19,8 +19,13 @@ the reference does (gs.py:397-398, mp.py:693) - the small dense
 eigenproblems and the blocked SVD / density-matrix eigendecompositions of
 the basis update (LAPACK; at most a few hundred rows per block).
 
-Not on this path: the ``omega`` excited-state target ((H - omega)^2 with a
-two-layer MPO), StackedMpo, on-the-fly site swapping and the primme solver.
+The ``omega`` target (the eigenpair nearest omega as the lowest state of
+(H - omega)^2, reference gs.py:106-111 with the two-layer environments and
+matvec of hop_expr.py:24-52) runs on the same single-layer kernels: the two
+layers of H - omega are merged into one MPO of squared bond dimension
+(``folded_mpo``), which gives the same contractions.
+
+Not on this path: StackedMpo, on-the-fly site swapping and the primme solver.
 """
 import logging
 from typing import List, Union
@@ -431,13 +436,37 @@ def _single_sweep(mps: Mps, mpo: Mpo, wt, environ: Environ, percent, last_opt_e_
     return micro, res_mps
 
 
+def folded_mpo(mpo: Mpo, omega: float) -> Mpo:
+    """(H - omega)^2 as one MPO.  The reference forms H - omega as
+    ``mpo.add(Mpo.identity(model).scale(-omega))`` (gs.py:109-110) and
+    contracts two layers of it, environments (bra, layer 1, layer 2, ket)
+    and the matvec "abcd, befg, cfhi, jgik, aej -> dhk" (hop_expr.py:30-39);
+    merging the two layers' bond indices, W2[(b,c), d, h, (g,i)] =
+    sum_f W[b,d,f,g] W[c,f,h,i], gives the same contractions with 3-leg
+    environments and the single-layer matvec, diagonal and dense kernels."""
+    ident = Mpo.identity(mpo.model)
+    ident._mp[ident.qnidx] = ident._mp[ident.qnidx] * (-float(omega))
+    shifted = mpo.add(ident)
+    folded = shifted.copy()
+    for i, w in enumerate(shifted._mp):
+        w = np.asarray(w, dtype=np.float64)
+        dl, d, _, dr = w.shape
+        w2 = np.einsum("bdfg,cfhi->bcdhgi", w, w)
+        folded._mp[i] = np.ascontiguousarray(w2.reshape(dl * dl, d, d, dr * dr))
+    folded.qn = [np.zeros((dim, mpo.model.qn_size), dtype=int) for dim in folded.bond_dims]
+    folded._device_cache = {}
+    return folded
+
+
 def optimize_mps(mps: Mps, mpo: Mpo, omega: float = None):
     """DMRG (reference gs.py:54-171).  ``mps`` is overwritten during the
-    optimisation.  Returns (sweep energies, optimised MPS or list of MPS)."""
-    if omega is not None:
-        raise NotImplementedError("the omega target ((H-omega)^2, two-layer MPO) is not on the device path")
+    optimisation.  Returns (sweep energies, optimised MPS or list of MPS);
+    with ``omega`` the energies are those of (H - omega)^2, as in the
+    reference."""
     if not isinstance(mpo, Mpo):
         raise NotImplementedError("StackedMpo is not on the device path")
+    if omega is not None:
+        mpo = folded_mpo(mpo, omega)
     cfg = mps.optimize_config
     assert cfg.method in ("2site", "1site")
     if cfg.algo not in ("davidson", "direct"):
