@@ -273,6 +273,32 @@ int reno_block_qr(int system, int rows, int cols, const void* coef,
                   int nblocks, const reno_qn_block* blocks, void* u, void* v,
                   int v_transposed, void* ws, size_t ws_bytes, void* stream);
 
+/* ---- svd_qn SVD branch (replaces svd_qn.py:99-240 with QR=False,
+ * full_matrices=True, the decomposition of mp.py:693 / _update_mps) ----
+ * Per block (caller's order == reference `set` iteration order), B_b =
+ * U_b S_b V_b^H with complete bases (rocSOLVER zgesvd).  u is (rows x
+ * sum_b nrows_b) and v (cols x sum_b ncols_b), complex128 row-major,
+ * zero-filled: first every block's min(nrows_b, ncols_b) singular columns,
+ * block by block, then every block's completion columns, block by block
+ * (blockappend of the full-matrices branch); v holds V_b without
+ * conjugation (the reference's v = vt.T).  s (device, float64) receives the
+ * singular values of every block in the same order. */
+size_t reno_svd_workspace_bytes(int rows, int cols, int nblocks, const reno_qn_block* blocks);
+int reno_block_svd(int rows, int cols, const void* coef, int nblocks, const reno_qn_block* blocks,
+                   void* u, void* v, void* s, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- eigh_qn (replaces svd_qn.py:243-303, the reduced-density-matrix
+ * decomposition of the state-averaged update, mp.py:769-826) -----------
+ * dm is the complex128 (n x n) density matrix.  For each block (square:
+ * rows == cols, a quantum-number sector, caller's order) the Hermitian part
+ * (B + B^H) / 2 is diagonalised (rocSOLVER zheev): u (n x sum_b nrows_b,
+ * row-major, zero-filled) receives the eigenvectors of every block at its
+ * row set and column offset, w (device, float64) the eigenvalues in
+ * ascending order within each block. */
+size_t reno_eigh_workspace_bytes(int n, int nblocks, const reno_qn_block* blocks);
+int reno_block_eigh(int n, const void* dm, int nblocks, const reno_qn_block* blocks, void* u, void* w,
+                    void* ws, size_t ws_bytes, void* stream);
+
 /* ---- DMRG local eigenproblem (FP64: L/R complex128, W float64) --------
  * Diagonal of the one-site H_eff, the Davidson preconditioner (replaces the
  * hdiag contraction of gs.py:424-436):

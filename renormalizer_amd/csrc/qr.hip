@@ -266,3 +266,202 @@ extern "C" int reno_block_qr(int system, int rows, int cols, const void* coef, i
   }
   return RENO_OK;
 }
+
+// ---- svd_qn SVD branch (full_matrices=True) --------------------------------
+// Per block B_b (nr x nc, gathered column-major), rocSOLVER zgesvd with both
+// bases complete: B = U S V^H.  u (rows x sum nr_b) and v (cols x sum nc_b)
+// take, as svd_qn lays them out (svd_qn.py:205-240 with full_matrices):
+// first every block's dim_b = min(nr_b, nc_b) singular columns, block by
+// block, then every block's remaining nr_b - dim_b (nc_b - dim_b) completion
+// columns, block by block; v holds V without conjugation (the reference's
+// v = vt.T, coef = u diag(s) v^T).  s receives the dim_b singular values of
+// each block in the same order (descending within a block).
+
+namespace reno {
+
+static size_t svd_block_ws_bytes(int nr, int nc) {
+  const int k = nr < nc ? nr : nc;
+  return align_up((size_t)nr * nc * 16) + align_up((size_t)nr * nr * 16) + align_up((size_t)nc * nc * 16) +
+         align_up((size_t)(k + 1) * sizeof(double)) + align_up(sizeof(int));
+}
+
+}  // namespace reno
+
+extern "C" size_t reno_svd_workspace_bytes(int rows, int cols, int nblocks, const reno_qn_block* blocks) {
+  (void)rows;
+  (void)cols;
+  size_t need = 0;
+  for (int b = 0; b < nblocks; ++b) need = std::max(need, svd_block_ws_bytes(blocks[b].nrows, blocks[b].ncols));
+  return need;
+}
+
+extern "C" int reno_block_svd(int rows, int cols, const void* coef, int nblocks, const reno_qn_block* blocks,
+                              void* u, void* v, void* s, void* ws, size_t ws_bytes, void* stream) {
+  if (rows <= 0 || cols <= 0 || !coef || !u || !v || !s || nblocks < 0 || (nblocks && !blocks)) {
+    set_error("block_svd: bad arguments");
+    return RENO_EINVAL;
+  }
+  if (nblocks == 0) {
+    set_error("block_svd: Invalid quantum number (no block)");
+    return RENO_EQN;
+  }
+  int64_t ut = 0, vt = 0, kt = 0;
+  for (int b = 0; b < nblocks; ++b) {
+    if (blocks[b].nrows <= 0 || blocks[b].ncols <= 0 || blocks[b].nrows > rows || blocks[b].ncols > cols ||
+        !blocks[b].rows || !blocks[b].cols) {
+      set_error("block_svd: bad block %d (%d x %d)", b, blocks[b].nrows, blocks[b].ncols);
+      return RENO_EINVAL;
+    }
+    ut += blocks[b].nrows;
+    vt += blocks[b].ncols;
+    kt += std::min(blocks[b].nrows, blocks[b].ncols);
+  }
+  const size_t need = reno_svd_workspace_bytes(rows, cols, nblocks, blocks);
+  if (!ws || ws_bytes < need) {
+    set_error("block_svd: workspace %zu < %zu", ws_bytes, need);
+    return RENO_ENOSPACE;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  rocblas_handle h;
+  RENO_TRY(get_handle(&h));
+  RENO_SOLVER_CHECK(rocblas_set_stream(h, st));
+  const double2* cf = (const double2*)coef;
+  double2* du = (double2*)u;
+  double2* dv = (double2*)v;
+  double* ds = (double*)s;
+  hipLaunchKernelGGL(k_zero_z, dim3(grid_for((int64_t)rows * ut)), dim3(256), 0, st, du, (int64_t)rows * ut);
+  hipLaunchKernelGGL(k_zero_z, dim3(grid_for((int64_t)cols * vt)), dim3(256), 0, st, dv, (int64_t)cols * vt);
+  RENO_CHECK_LAUNCH();
+  int64_t off_s = 0;           // singular columns (and s) of the blocks so far
+  int64_t off_eu = kt, off_ev = kt;  // completion columns so far
+  for (int ib = 0; ib < nblocks; ++ib) {
+    const reno_qn_block& b = blocks[ib];
+    const int nr = b.nrows, nc = b.ncols;
+    const int k = std::min(nr, nc);
+    char* p = (char*)ws;
+    double2* X = (double2*)p;
+    p += align_up((size_t)nr * nc * 16);
+    double2* U = (double2*)p;
+    p += align_up((size_t)nr * nr * 16);
+    double2* VH = (double2*)p;
+    p += align_up((size_t)nc * nc * 16);
+    double* E = (double*)p;
+    p += align_up((size_t)(k + 1) * sizeof(double));
+    int* info = (int*)p;
+    hipLaunchKernelGGL(k_gather, dim3(grid_for((int64_t)nr * nc)), dim3(256), 0, st, cf, (int64_t)cols, b.rows,
+                       b.cols, nr, nc, 0, X);
+    RENO_CHECK_LAUNCH();
+    RENO_SOLVER_CHECK(rocsolver_zgesvd(h, rocblas_svect_all, rocblas_svect_all, nr, nc,
+                                       (rocblas_double_complex*)X, nr, ds + off_s, (rocblas_double_complex*)U,
+                                       nr, (rocblas_double_complex*)VH, nc, E, rocblas_outofplace, info));
+    // u[rows_b, off_s + j] = U[:, j] (j < k); u[rows_b, off_eu + j - k] = U[:, j] (j >= k)
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nr * k)), dim3(256), 0, st, (const double2*)U,
+                       (int64_t)0, (int64_t)1, (int64_t)nr, b.rows, nr, k, (int64_t)ut, (int64_t)1, off_s, 2, 0, 0,
+                       du);
+    if (nr > k)
+      hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nr * (nr - k))), dim3(256), 0, st,
+                         (const double2*)U, (int64_t)k * nr, (int64_t)1, (int64_t)nr, b.rows, nr, nr - k,
+                         (int64_t)ut, (int64_t)1, off_eu, 2, 0, 0, du);
+    // v[cols_b, off + j] = VH[j, c] (the reference's v = vt.T)
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nc * k)), dim3(256), 0, st, (const double2*)VH,
+                       (int64_t)0, (int64_t)nc, (int64_t)1, b.cols, nc, k, (int64_t)vt, (int64_t)1, off_s, 2, 0, 0,
+                       dv);
+    if (nc > k)
+      hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nc * (nc - k))), dim3(256), 0, st,
+                         (const double2*)VH, (int64_t)k, (int64_t)nc, (int64_t)1, b.cols, nc, nc - k, (int64_t)vt,
+                         (int64_t)1, off_ev, 2, 0, 0, dv);
+    RENO_CHECK_LAUNCH();
+    off_s += k;
+    off_eu += nr - k;
+    off_ev += nc - k;
+  }
+  return RENO_OK;
+}
+
+// ---- eigh_qn: per quantum-number sector of a reduced density matrix --------
+namespace reno {
+
+// X (n x n, column-major) <- (X + X^H) / 2
+__global__ void k_hermitize(double2* X, int n) {
+  const int64_t total = (int64_t)n * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = t / n, i = t - j * n;
+    if (i < j) continue;  // each pair once, by its lower element
+    const double2 a = X[i + j * n], b = X[j + i * n];
+    const double2 lo = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+    X[i + j * n] = lo;
+    X[j + i * n] = make_double2(lo.x, -lo.y);
+  }
+}
+
+static size_t eigh_block_ws_bytes(int nb) {
+  return align_up((size_t)nb * nb * 16) + align_up((size_t)(nb + 1) * sizeof(double)) + align_up(sizeof(int));
+}
+
+}  // namespace reno
+
+extern "C" size_t reno_eigh_workspace_bytes(int n, int nblocks, const reno_qn_block* blocks) {
+  (void)n;
+  size_t need = 0;
+  for (int b = 0; b < nblocks; ++b) need = std::max(need, eigh_block_ws_bytes(blocks[b].nrows));
+  return need;
+}
+
+extern "C" int reno_block_eigh(int n, const void* dm, int nblocks, const reno_qn_block* blocks, void* u, void* w,
+                               void* ws, size_t ws_bytes, void* stream) {
+  if (n <= 0 || !dm || !u || !w || nblocks < 0 || (nblocks && !blocks)) {
+    set_error("block_eigh: bad arguments");
+    return RENO_EINVAL;
+  }
+  if (nblocks == 0) {
+    set_error("block_eigh: Invalid quantum number (no block)");
+    return RENO_EQN;
+  }
+  int64_t ut = 0;
+  for (int b = 0; b < nblocks; ++b) {
+    if (blocks[b].nrows <= 0 || blocks[b].nrows != blocks[b].ncols || blocks[b].nrows > n || !blocks[b].rows ||
+        !blocks[b].cols) {
+      set_error("block_eigh: bad block %d (%d x %d)", b, blocks[b].nrows, blocks[b].ncols);
+      return RENO_EINVAL;
+    }
+    ut += blocks[b].nrows;
+  }
+  const size_t need = reno_eigh_workspace_bytes(n, nblocks, blocks);
+  if (!ws || ws_bytes < need) {
+    set_error("block_eigh: workspace %zu < %zu", ws_bytes, need);
+    return RENO_ENOSPACE;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  rocblas_handle h;
+  RENO_TRY(get_handle(&h));
+  RENO_SOLVER_CHECK(rocblas_set_stream(h, st));
+  double2* du = (double2*)u;
+  double* dw = (double*)w;
+  hipLaunchKernelGGL(k_zero_z, dim3(grid_for((int64_t)n * ut)), dim3(256), 0, st, du, (int64_t)n * ut);
+  RENO_CHECK_LAUNCH();
+  int64_t off = 0;
+  for (int ib = 0; ib < nblocks; ++ib) {
+    const reno_qn_block& b = blocks[ib];
+    const int nb = b.nrows;
+    char* p = (char*)ws;
+    double2* X = (double2*)p;
+    p += align_up((size_t)nb * nb * 16);
+    double* E = (double*)p;
+    p += align_up((size_t)(nb + 1) * sizeof(double));
+    int* info = (int*)p;
+    hipLaunchKernelGGL(k_gather, dim3(grid_for((int64_t)nb * nb)), dim3(256), 0, st, (const double2*)dm,
+                       (int64_t)n, b.rows, b.cols, nb, nb, 0, X);
+    hipLaunchKernelGGL(k_hermitize, dim3(grid_for((int64_t)nb * nb)), dim3(256), 0, st, X, nb);
+    RENO_CHECK_LAUNCH();
+    RENO_SOLVER_CHECK(rocsolver_zheev(h, rocblas_evect_original, rocblas_fill_lower, nb,
+                                       (rocblas_double_complex*)X, nb, dw + off, E, info));
+    // u[rows_b, off + j] = X[:, j]
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nb * nb)), dim3(256), 0, st, (const double2*)X,
+                       (int64_t)0, (int64_t)1, (int64_t)nb, b.rows, nb, nb, (int64_t)ut, (int64_t)1, off, 2, 0, 0,
+                       du);
+    RENO_CHECK_LAUNCH();
+    off += nb;
+  }
+  return RENO_OK;
+}

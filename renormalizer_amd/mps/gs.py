@@ -197,9 +197,36 @@ def _blocked_svd(cmat, qnbigl, qnbigr, qntot):
     quantum-number block (reference block order) A_b = U S Vt, returned as
     U (rows x .), S_U, qn_U, V (cols x .) with V = Vt^T, S_V, qn_V; each side
     lists the singular vectors of all blocks, then the zero-weight
-    completions.  The blocks are small: LAPACK on the host, as the reference
-    (mp.py:693); the factors go back to the device."""
-    a = cmat.cpu().numpy()
+    completions.  Device matrices are factored on the device
+    (native.block_svd, rocSOLVER); only the singular values come to the host
+    for the basis selection."""
+    if cmat.is_cuda:
+        return _blocked_svd_device(cmat, qnbigl, qnbigr, qntot)
+    return _blocked_svd_host(cmat, qnbigl, qnbigr, qntot)
+
+
+def _blocked_svd_device(cmat, qnbigl, qnbigr, qntot):
+    blocks = qn_blocks(qnbigl, qnbigr, qntot)
+    if not blocks:
+        raise ValueError("Invalid quantum number")
+    U, V, s = native.block_svd(cmat.contiguous(), [(ls, rs) for _, _, ls, rs in blocks])
+    s = s.cpu().numpy()
+    if not np.isfinite(s).all() or (s < 0).any():
+        raise FloatingPointError("device SVD did not converge")
+    qnl, qnl0, qnr, qnr0 = [], [], [], []
+    for nl, nr, ls, rs in blocks:
+        dim = min(len(ls), len(rs))
+        qnl += [tuple(nl)] * dim
+        qnr += [tuple(nr)] * dim
+        qnl0 += [tuple(nl)] * (len(ls) - dim)
+        qnr0 += [tuple(nr)] * (len(rs) - dim)
+    SU = np.concatenate([s, np.zeros(len(qnl0))])
+    SV = np.concatenate([s, np.zeros(len(qnr0))])
+    return U, SU, qnl + qnl0, V, SV, qnr + qnr0
+
+
+def _blocked_svd_host(cmat, qnbigl, qnbigr, qntot):
+    a = cmat.cpu().numpy() if isinstance(cmat, torch.Tensor) else np.asarray(cmat)
     if not np.isfinite(a).all():
         raise FloatingPointError("non-finite coefficient matrix")
     rows, cols = a.shape
@@ -224,7 +251,7 @@ def _blocked_svd(cmat, qnbigl, qnbigr, qntot):
             sl0.append(np.zeros(extra))
     if not u_l:
         raise ValueError("Invalid quantum number")
-    dev = cmat.device
+    dev = cmat.device if isinstance(cmat, torch.Tensor) else torch.device("cpu")
     U = torch.from_numpy(np.concatenate(u_l + u0_l, axis=1)).to(dev)
     V = torch.from_numpy(np.concatenate(v_l + v0_l, axis=1)).to(dev)
     SU = np.concatenate(s_l + su0)
@@ -234,11 +261,27 @@ def _blocked_svd(cmat, qnbigl, qnbigr, qntot):
 
 def _blocked_dm_eigh(ddm, qnbig, comp_qnbig, qntot):
     """eigh_qn (reference svd_qn.py:243-303): per sector of the reduced
-    density matrix, eigenvectors and sqrt of the (clipped) eigenvalues
-    (host LAPACK; eigenvectors back on the device)."""
+    density matrix, eigenvectors and sqrt of the (clipped) eigenvalues.
+    Device matrices are diagonalised on the device (native.block_eigh,
+    rocSOLVER); only the eigenvalues come to the host."""
     qn_size = len(qntot)
     localqn = qnbig.reshape(-1, qn_size)
-    dm = ddm.cpu().numpy()
+    if isinstance(ddm, torch.Tensor) and ddm.is_cuda:
+        sectors, new_qn = [], []
+        for nl in set([tuple(t) for t in localqn]):
+            nr = qntot - np.array(nl)
+            if np.sum(get_qn_mask(comp_qnbig, nr)) == 0:
+                continue
+            lset = np.where(get_qn_mask(localqn, nl))[0]
+            sectors.append(lset)
+            new_qn += [nl] * len(lset)
+        u, w = native.block_eigh(ddm.contiguous(), sectors)
+        s2 = w.cpu().numpy()
+        if not np.isfinite(s2).all() or not bool(torch.isfinite(torch.view_as_real(u)).all()):
+            raise FloatingPointError("device eigendecomposition returned non-finite values")
+        s2[s2 < 0] = 0
+        return u, np.sqrt(s2), new_qn
+    dm = ddm.cpu().numpy() if isinstance(ddm, torch.Tensor) else np.asarray(ddm)
     u_l, s_l, new_qn = [], [], []
     for nl in set([tuple(t) for t in localqn]):
         nr = qntot - np.array(nl)
@@ -253,7 +296,8 @@ def _blocked_dm_eigh(ddm, qnbig, comp_qnbig, qntot):
         rec[lset] = bu
         u_l.append(rec)
         new_qn += [nl] * len(lset)
-    return torch.from_numpy(np.concatenate(u_l, axis=1)).to(ddm.device), np.concatenate(s_l), new_qn
+    dev = ddm.device if isinstance(ddm, torch.Tensor) else torch.device("cpu")
+    return torch.from_numpy(np.concatenate(u_l, axis=1)).to(dev), np.concatenate(s_l), new_qn
 
 
 def _take_columns(mat, sidx, scale=None):

@@ -34,6 +34,7 @@ EXPORTS = (
     "reno_env_workspace_bytes", "reno_qr_workspace_bytes",
     "reno_hop_apply", "reno_hop_apply_noalloc", "reno_env_update", "reno_env_close", "reno_expm_krylov",
     "reno_expm_krylov_async", "reno_expm_krylov_mixed_async", "reno_block_qr", "reno_heff_diag", "reno_heff_dense", "reno_zgemm",
+    "reno_svd_workspace_bytes", "reno_block_svd", "reno_eigh_workspace_bytes", "reno_block_eigh",
     "reno_batch_scratch_bytes", "reno_batch_site_step", "reno_batch_normalize",
     "reno_autotune", "reno_autotune_reset", "reno_autotune_entries",
     "reno_probe_mfma16", "reno_prof_enable", "reno_prof_collect", "reno_prof_kind_info",
@@ -134,6 +135,15 @@ def load_library(path: str = LIB_PATH):
         lib.reno_block_qr.restype = c_int
         lib.reno_block_qr.argtypes = [c_int, c_int, c_int, vp, c_int, ctypes.POINTER(QnBlock),
                                       vp, vp, c_int, vp, c_size, vp]
+        lib.reno_svd_workspace_bytes.restype = c_size
+        lib.reno_svd_workspace_bytes.argtypes = [c_int, c_int, c_int, ctypes.POINTER(QnBlock)]
+        lib.reno_block_svd.restype = c_int
+        lib.reno_block_svd.argtypes = [c_int, c_int, vp, c_int, ctypes.POINTER(QnBlock), vp, vp, vp,
+                                       vp, c_size, vp]
+        lib.reno_eigh_workspace_bytes.restype = c_size
+        lib.reno_eigh_workspace_bytes.argtypes = [c_int, c_int, ctypes.POINTER(QnBlock)]
+        lib.reno_block_eigh.restype = c_int
+        lib.reno_block_eigh.argtypes = [c_int, vp, c_int, ctypes.POINTER(QnBlock), vp, vp, vp, c_size, vp]
         lib.reno_env_close.restype = c_int
         lib.reno_env_close.argtypes = [c_int, c_int, c_int, c_int, vp, vp, vp, vp]
         lib.reno_zgemm.restype = c_int
@@ -559,6 +569,58 @@ def block_qr(system, coef, blocks, v_transposed=False):
     check(l.reno_block_qr(sysc, rows, cols, _ptr(coef), len(blocks), cblocks, _ptr(u), _ptr(v),
                           int(bool(v_transposed)), _ptr(ws), nbytes, _stream()), "reno_block_qr")
     return u, v
+
+
+def block_svd(coef, blocks):
+    """QN-blocked SVD with complete bases (svd_qn's full_matrices branch) of
+    a (rows x cols) complex128 device matrix.  ``blocks`` is a list of
+    (row_index_array, col_index_array) in the reference's block order.
+    Returns (u, v, s) on the device: u (rows x sum nrows), v (cols x sum
+    ncols), first the singular columns of every block then the
+    completions, and s the singular values of every block."""
+    _require_device(coef)
+    if coef.dtype != torch.complex128 or coef.dim() != 2 or not coef.is_contiguous():
+        raise ValueError("coefficient matrix must be a contiguous 2-D complex128 tensor")
+    rows, cols = coef.shape
+    if len(blocks) == 0:
+        raise ValueError("Invalid quantum number")
+    dev = coef.device
+    cblocks, keep, kt = qn_block_descs(blocks, dev)
+    ut = sum(len(r) for r, _ in blocks)
+    vt = sum(len(c) for _, c in blocks)
+    u = torch.empty((rows, ut), dtype=torch.complex128, device=dev)
+    v = torch.empty((cols, vt), dtype=torch.complex128, device=dev)
+    s = torch.empty((kt,), dtype=torch.float64, device=dev)
+    l = lib()
+    nbytes = l.reno_svd_workspace_bytes(rows, cols, len(blocks), cblocks)
+    ws = workspace.get(nbytes, dev)
+    check(l.reno_block_svd(rows, cols, _ptr(coef), len(blocks), cblocks, _ptr(u), _ptr(v), _ptr(s),
+                           _ptr(ws), nbytes, _stream()), "reno_block_svd")
+    return u, v, s
+
+
+def block_eigh(dm, sectors):
+    """Eigen-decomposition of the Hermitian parts of the diagonal blocks
+    ``sectors`` (row index arrays) of a complex128 (n x n) device matrix.
+    Returns (u, w) on the device: u (n x sum len) with every block's
+    eigenvectors at its rows, w the eigenvalues, ascending per block."""
+    _require_device(dm)
+    if dm.dtype != torch.complex128 or dm.dim() != 2 or dm.shape[0] != dm.shape[1] or not dm.is_contiguous():
+        raise ValueError("density matrix must be a contiguous square complex128 tensor")
+    if len(sectors) == 0:
+        raise ValueError("Invalid quantum number")
+    n = dm.shape[0]
+    dev = dm.device
+    cblocks, keep, _ = qn_block_descs([(r, r) for r in sectors], dev)
+    ut = sum(len(r) for r in sectors)
+    u = torch.empty((n, ut), dtype=torch.complex128, device=dev)
+    w = torch.empty((ut,), dtype=torch.float64, device=dev)
+    l = lib()
+    nbytes = l.reno_eigh_workspace_bytes(n, len(sectors), cblocks)
+    ws = workspace.get(nbytes, dev)
+    check(l.reno_block_eigh(n, _ptr(dm), len(sectors), cblocks, _ptr(u), _ptr(w), _ptr(ws), nbytes,
+                            _stream()), "reno_block_eigh")
+    return u, w
 
 
 def qn_block_descs(blocks, dev):
