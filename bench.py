@@ -43,11 +43,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_workload(nmode=64, nlev=10, m=256, cache=None):
-    """The C4 model, its MPO and the expanded initial state.  With ``cache``
-    (a .npz path) the initial state is loaded from there when the file
-    exists (Mps.load: the same tensors and quantum numbers bit for bit) and
-    written there after the expansion otherwise."""
+def build_workload(nmode=64, nlev=10, m=256, cache=None, on_device=True):
+    """The C4 model, its MPO and the expanded initial state, expanded on the
+    device (``on_device``) or on the host.  With ``cache`` (a .npz path) the
+    initial state is loaded from there when the file exists (Mps.load: the
+    same tensors and quantum numbers bit for bit) and written there after
+    the expansion otherwise."""
     from renormalizer_amd import (CompressConfig, CompressCriteria, EvolveConfig, EvolveMethod, Mpo, Mps,
                                   Phonon, Quantity, SpinBosonModel)
     from renormalizer_amd.sbm import SpectralDensityFunction
@@ -63,6 +64,8 @@ def build_workload(nmode=64, nlev=10, m=256, cache=None):
         mps = Mps.ground_state(model, False)
         mps.compress_config = CompressConfig(CompressCriteria.fixed, max_bonddim=m)
         mps.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps)
+        if on_device:
+            mps.to_device()
         mps = mps.expand_bond_dimension(mpo, coef=1e-16, include_ex=False)
         if cache:
             mps.dump(cache)
@@ -351,7 +354,7 @@ def cpu_trajectory(workload, steps):
         mps, mpo = helpers.holstein_initial(model, 64)
         dt = 0.02
     elif workload == "c4":
-        _, mpo, mps = build_workload(64, 10, 256)
+        _, mpo, mps = build_workload(64, 10, 256, on_device=False)
         dt = 0.025
     else:
         raise SystemExit(f"unknown workload {workload}")
@@ -654,6 +657,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--torch-steps", type=int, default=2,
                     help="timed steps of the plain-torch FP64 baseline (0: skip it)")
+    ap.add_argument("--host-start", action=argparse.BooleanOptionalAction, default=True,
+                    help="also time both modes from the start built by the host (numpy/LAPACK) "
+                         "expansion, the reference's construction (host_start in the output)")
     ap.add_argument("--state-cache", metavar="NPZ", default=None,
                     help="load the initial state from this file if it exists, else build and "
                          "write it there (profiling runs)")
@@ -718,7 +724,9 @@ def main():
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
     model, mpo, mps0 = build_workload(a.modes, 10, a.bond_dim, cache=a.state_cache)
-    log(f"initial state (host expansion) {time.perf_counter() - t0:.1f} s, bond dims max "
+    torch.cuda.synchronize()
+    init_s = time.perf_counter() - t0
+    log(f"initial state (device expansion) {init_s:.1f} s, bond dims max "
         f"{max(mps0.bond_dims)}, sites {len(mps0)}")
 
     res = {}
@@ -768,6 +776,7 @@ def main():
                    "autotune": bool(a.autotune)},
         "roofline": rl,
         "krylov_steps_redone": r["redone"],
+        "initial_state_s": round(init_s, 2),
     }
     if err is not None:
         out["fp64_sweeps_per_s"] = round(res["fp64"]["steps_per_s"], 4)
@@ -784,6 +793,29 @@ def main():
         out["mixed_fp32_matvec_share"] = round(sum(res["mixed"]["fp32_matvecs_per_step"]) / mvmx, 3)
         other = "mixed" if main_mode == "fp64" else "fp64"
         out[other + "_roofline"] = roofline(res[other]["prof"], res[other]["seconds"], a.steps)
+    if a.host_start and a.state_cache is None:
+        # The admixture directions of the 1e-16 expansion are fixed by rounding:
+        # the host's LAPACK SVD of the graded matrix draws them from rounding
+        # noise, the device's QR + Gram eigensolver keeps them inside the span
+        # of H^k psi.  Same physics (<sigma_z> within the hosts' own gesdd /
+        # gesvd spread), different Lanczos work per step: both are reported.
+        t1 = time.perf_counter()
+        _, _, mps_h = build_workload(a.modes, 10, a.bond_dim, on_device=False)
+        hs = {"build_s": round(time.perf_counter() - t1, 1)}
+        log(f"host-built start in {hs['build_s']} s")
+        for prec in res:
+            rh = run_mode(mps_h, mpo, prec, a.dt, a.warmup, a.steps, profile=False, tau=a.mixed_tau,
+                          autotune=a.autotune)
+            hs[prec + "_sweeps_per_s"] = round(rh["steps_per_s"], 4)
+            hs[prec + "_matvecs_per_step"] = rh["matvecs_per_step"]
+            hs[prec + "_max_abs_sigma_z_diff_vs_device_start"] = float(
+                np.max(np.abs(np.array(rh["sz"]) - np.array(res[prec]["sz"]))))
+            log(f"host start {prec}: {rh['steps_per_s']:.4f} sweeps/s, matvecs/step {rh['matvecs_per_step']}")
+        for prec in res:
+            hs["device_start_" + prec + "_matvecs_per_step"] = res[prec]["matvecs_per_step"]
+        hs["what"] = ("the same steps from the start expanded on the host (numpy/LAPACK SVD, the reference's "
+                      "construction); value above is from the device-expanded start")
+        out["host_start"] = hs
     if a.torch_steps > 0 and "fp64" in res:
         tb = torch_baseline(mps0, mpo, a.dt, a.torch_steps, res["fp64"]["sz"])
         out["torch_fp64_baseline"] = tb

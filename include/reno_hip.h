@@ -291,7 +291,9 @@ int reno_block_svd(int rows, int cols, const void* coef, int nblocks, const reno
  * decomposition of the state-averaged update, mp.py:769-826) -----------
  * dm is the complex128 (n x n) density matrix.  For each block (square:
  * rows == cols, a quantum-number sector, caller's order) the Hermitian part
- * (B + B^H) / 2 is diagonalised (rocSOLVER zheev): u (n x sum_b nrows_b,
+ * (B + B^H) / 2 is diagonalised (rocSOLVER zheevdj; a block whose result is
+ * not finite is solved again with zheev, so the call waits for the stream
+ * once per block): u (n x sum_b nrows_b,
  * row-major, zero-filled) receives the eigenvectors of every block at its
  * row set and column offset, w (device, float64) the eigenvalues in
  * ascending order within each block. */

@@ -396,7 +396,26 @@ __global__ void k_hermitize(double2* X, int n) {
 }
 
 static size_t eigh_block_ws_bytes(int nb) {
-  return align_up((size_t)nb * nb * 16) + align_up((size_t)(nb + 1) * sizeof(double)) + align_up(sizeof(int));
+  return align_up((size_t)nb * nb * 16) + align_up((size_t)(nb + 1) * sizeof(double)) + align_up(2 * sizeof(int));
+}
+
+// flag[0] = 1 when the eigensolver reported failure (info != 0) or left a
+// non-finite eigenvalue or eigenvector entry, else 0.  One workgroup.
+__global__ __launch_bounds__(256) void k_eigh_bad(const double2* __restrict__ X, int64_t nx,
+                                                  const double* __restrict__ w, int nw,
+                                                  const int* __restrict__ info, int* flag) {
+  __shared__ int any;
+  if (threadIdx.x == 0) any = *info != 0;
+  __syncthreads();
+  int bad = 0;
+  for (int64_t i = threadIdx.x; i < nx; i += blockDim.x) {
+    const double2 v = X[i];
+    bad |= !(isfinite(v.x) && isfinite(v.y));
+  }
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) bad |= !isfinite(w[i]);
+  if (bad) any = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) flag[0] = any;
 }
 
 }  // namespace reno
@@ -450,12 +469,34 @@ extern "C" int reno_block_eigh(int n, const void* dm, int nblocks, const reno_qn
     double* E = (double*)p;
     p += align_up((size_t)(nb + 1) * sizeof(double));
     int* info = (int*)p;
-    hipLaunchKernelGGL(k_gather, dim3(grid_for((int64_t)nb * nb)), dim3(256), 0, st, (const double2*)dm,
-                       (int64_t)n, b.rows, b.cols, nb, nb, 0, X);
-    hipLaunchKernelGGL(k_hermitize, dim3(grid_for((int64_t)nb * nb)), dim3(256), 0, st, X, nb);
+    int* flag = info + 1;
+    auto load = [&]() {
+      hipLaunchKernelGGL(k_gather, dim3(grid_for((int64_t)nb * nb)), dim3(256), 0, st, (const double2*)dm,
+                         (int64_t)n, b.rows, b.cols, nb, nb, 0, X);
+      hipLaunchKernelGGL(k_hermitize, dim3(grid_for((int64_t)nb * nb)), dim3(256), 0, st, X, nb);
+    };
+    load();
     RENO_CHECK_LAUNCH();
-    RENO_SOLVER_CHECK(rocsolver_zheev(h, rocblas_evect_original, rocblas_fill_lower, nb,
-                                       (rocblas_double_complex*)X, nb, dw + off, E, info));
+    // Jacobi eigenvalues, divide-and-conquer eigenvectors; rocSOLVER's
+    // zheevd was measured returning NaN eigenvectors with info = 0 for
+    // low-rank Gram matrices (eigenvalues 1 down to 1e-47), so every result
+    // is checked on the device and a block that is not finite is solved
+    // again by QR iteration (zheev).  The check waits for the stream once
+    // per block.
+    RENO_SOLVER_CHECK(rocsolver_zheevdj(h, rocblas_evect_original, rocblas_fill_lower, nb,
+                                        (rocblas_double_complex*)X, nb, dw + off, info));
+    hipLaunchKernelGGL(k_eigh_bad, dim3(1), dim3(256), 0, st, (const double2*)X, (int64_t)nb * nb,
+                       (const double*)(dw + off), nb, (const int*)info, flag);
+    RENO_CHECK_LAUNCH();
+    int host_flag = 0;
+    RENO_HIP_CHECK(hipMemcpyAsync(&host_flag, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    RENO_HIP_CHECK(hipStreamSynchronize(st));
+    if (host_flag) {
+      load();
+      RENO_CHECK_LAUNCH();
+      RENO_SOLVER_CHECK(rocsolver_zheev(h, rocblas_evect_original, rocblas_fill_lower, nb,
+                                         (rocblas_double_complex*)X, nb, dw + off, E, info));
+    }
     // u[rows_b, off + j] = X[:, j]
     hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)nb * nb)), dim3(256), 0, st, (const double2*)X,
                        (int64_t)0, (int64_t)1, (int64_t)nb, b.rows, nb, nb, (int64_t)ut, (int64_t)1, off, 2, 0, 0,

@@ -7,9 +7,10 @@ or on the device (torch complex128, after ``to_device``).  The bookkeeping
 (qn, qnidx, qntot, to_right) always stays on the host as small int arrays,
 exactly as the reference stores it.
 
-The host-side algorithms in this file (add, canonicalise, compress, dot on
-numpy tensors) are initial-state preparation only; the time-evolution and
-measurement path runs on the device (mps.py, lib.py).
+The setup algorithms in this file (add, canonicalise, compress) run on
+host numpy tensors here and on device tensors in device_setup.py (states
+only); the time-evolution and measurement path runs on the device (mps.py,
+lib.py).
 """
 from typing import List
 
@@ -206,10 +207,12 @@ class MatrixProduct:
             raise RuntimeError(f"{what} is an initial-state preparation step on host tensors; "
                                f"call it before the state is uploaded")
 
-    # ---- host-side setup algorithms (reference mp.py:374-511, 890-1007) --
+    # ---- setup algorithms (reference mp.py:374-511, 890-1007) -----------
     def add(self, other: "MatrixProduct"):
         """Direct sum of the bond spaces (reference mp.py:374-435)."""
-        self._require_host("add")
+        if self.on_device or other.on_device:
+            from renormalizer_amd.mps import device_setup
+            return device_setup.add(self, other)
         assert np.all(self.qntot == other.qntot)
         assert self.site_num == other.site_num
         new = self.metacopy()
@@ -328,7 +331,9 @@ class MatrixProduct:
 
     def canonicalise(self, stop_idx: int = None):
         """QR sweep moving the canonical centre (reference mp.py:890-922)."""
-        self._require_host("canonicalise")
+        if self.on_device:
+            from renormalizer_amd.mps import device_setup
+            return device_setup.canonicalise(self, stop_idx)
         if self.to_right:
             assert self.qnidx == 0
         else:
@@ -381,7 +386,9 @@ class MatrixProduct:
 
     def compress(self, temp_m_trunc=None, ret_s=False):
         """SVD truncation sweep (reference mp.py:437-511)."""
-        self._require_host("compress")
+        if self.on_device:
+            from renormalizer_amd.mps import device_setup
+            return device_setup.compress(self, temp_m_trunc, ret_s)
         if self.to_right:
             assert self.qnidx == 0
         else:

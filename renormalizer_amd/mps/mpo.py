@@ -117,9 +117,14 @@ class Mpo(MatrixProduct):
             return cached[key]
 
     def apply(self, mp: MatrixProduct, canonicalise: bool = False) -> MatrixProduct:
-        """MPO x MPS on host tensors (initial-state preparation; reference
-        mpo.py:331-389)."""
-        mp._require_host("Mpo.apply")
+        """MPO x MPS (initial-state preparation; reference mpo.py:331-389),
+        on the device for a device state."""
+        if mp.on_device:
+            from renormalizer_amd.mps import device_setup
+            new = device_setup.mpo_apply(self, mp)
+            if canonicalise:
+                new.canonicalise()
+            return new
         assert self.site_num == mp.site_num
         new = mp.copy()
         if np.iscomplexobj(self._mp[0]) and not new.is_complex:

@@ -863,13 +863,21 @@ def expand_bond_dimension(mps: Mps, hint_mpo=None, coef=1e-10, include_ex=True):
         ex_state.compress_config = mps.compress_config
         ex_state.move_qnidx(mps.qnidx)
         ex_state.to_right = mps.to_right
+        if mps.on_device:
+            from renormalizer_amd.mps.device_setup import upload_complex
+            upload_complex(ex_state, mps[0].device)
     else:
         ex_state = None
     return expand_bond_dimension_general(mps, hint_mpo, coef, ex_state)
 
 
 def expand_bond_dimension_general(mps, hint_mpo=None, coef=1e-10, ex_mps=None):
-    mps._require_host("expand_bond_dimension")
+    """Reference mps.py:1963-2023.  Runs where the state is: a device state
+    (complex128 sites) is expanded with the device operations of
+    device_setup.py, a host state with the numpy ones of mp.py."""
+    if mps.on_device:
+        from renormalizer_amd.mps.device_setup import upload_complex
+        mps = upload_complex(mps.copy(), mps[0].device)
     mps.compress_config.set_bonddim(len(mps.bond_dims))
     m_target = np.minimum(np.array(mps.compress_config.max_dims) - np.array(mps.bond_dims),
                           mps.bond_dims_exact)

@@ -5,15 +5,18 @@
   reference (svd_qn.py:177), so block layout and new bond quantum numbers are
   bit-identical.
 * ``svd_qn_device`` - the QR/RQ branch (QR=True, economic) on the device
-  through ``reno_block_qr`` (rocSOLVER Householder factorisations).
+  through ``reno_block_qr`` (Householder factorisations).
+* ``svd_qn_device_svd`` - the SVD branch (full_matrices=False) on the
+  device: block QR, then the Hermitian eigensolver on the k x k factor.
 * ``svd_qn_host`` - the full routine (SVD and QR branches) in host numpy,
-  used only while preparing initial states (canonicalise / compress of the
-  starting MPS) before the state is uploaded.
+  for states prepared on the host (canonicalise / compress of a starting
+  MPS before it is uploaded).
 """
 import threading
 
 import numpy as np
 import scipy.linalg
+import torch
 
 from renormalizer_amd import native
 
@@ -91,6 +94,72 @@ def svd_qn_device(coef, qnbigl, qnbigr, qntot, system, plan=None):
     rows, cols, blocks, qnl, qnr = plan
     u, vt = native.block_qr(system, coef.reshape(rows, cols), blocks, v_transposed=True)
     return u, list(qnl), vt, list(qnr)
+
+
+def _ct(t):
+    return t.conj().transpose(0, 1).contiguous()
+
+
+def svd_qn_device_svd(coef, qnbigl, qnbigr, qntot, system, plan=None):
+    """SVD branch of svd_qn with full_matrices=False (reference
+    svd_qn.py:205-240) on the device, for the truncation sweeps of
+    ``compress``.
+
+    Per quantum-number block the QR ("L") or RQ ("R") factorisation
+    (reno_block_qr) leaves a k x k factor T beside an orthonormal factor Q.
+    The Hermitian eigen-decomposition (reno_block_eigh) of T T^H ("L") or
+    T^H T ("R") gives the unitary W that carries Q onto the singular vectors:
+    U = Q W and sigma V^H = W^H T ("L"), or V^H = W^H Q and U sigma = T W
+    ("R").  The singular values are the norms of the rows ("L") or columns
+    ("R") of the rotated factor, which holds them to the absolute accuracy
+    eps * sigma_max of a bidiagonal SVD.  Singular directions whose sigma
+    lies below about sqrt(eps) * sigma_max are resolved only up to a
+    rotation among themselves (the Gram matrix squares them); the span they
+    share with the larger ones is exact to rounding.
+
+    Returns (site, sigma, qn, rest) in the reference's order (numpy argsort
+    of sigma, descending): "L": site = U (rows x k) and rest = sigma V^H
+    (k x cols); "R": site = V^H (k x cols) and rest = U sigma (rows x k);
+    sigma a host float64 array and qn the new bond quantum numbers (the
+    reference's new_qnl for "L", new_qnr for "R")."""
+    if plan is None:
+        plan = svd_qn_plan(qnbigl, qnbigr, qntot)
+    rows, cols, blocks, qnl, qnr = plan
+    first, second = native.block_qr(system, coef.reshape(rows, cols), blocks, v_transposed=True)
+    sectors = []
+    off = 0
+    for r, c in blocks:
+        k = min(len(r), len(c))
+        sectors.append(np.arange(off, off + k, dtype=np.int64))
+        off += k
+    if system == "L":
+        q, t = first, second  # coef = q @ t, t (k x cols)
+        w, _ = native.block_eigh(native.zgemm(t, _ct(t)), sectors)
+        site = native.zgemm(q, w)
+        rest = native.zgemm(_ct(w), t)
+        sigma_dev = torch.linalg.vector_norm(rest, dim=1)
+        qn = qnl
+    elif system == "R":
+        t, q = first, second  # coef = t @ q, t (rows x k)
+        w, _ = native.block_eigh(native.zgemm(_ct(t), t), sectors)
+        site = native.zgemm(_ct(w), q)
+        rest = native.zgemm(t, w)
+        sigma_dev = torch.linalg.vector_norm(rest, dim=0)
+        qn = qnr
+    else:
+        raise AssertionError("system")
+    sigma = sigma_dev.cpu().numpy()
+    if not np.all(np.isfinite(sigma)):
+        raise FloatingPointError("svd_qn_device_svd: non-finite singular values")
+    order = np.argsort(sigma)[::-1].copy()
+    idx = torch.from_numpy(order).to(coef.device)
+    if system == "L":
+        site = site.index_select(1, idx)
+        rest = rest.index_select(0, idx)
+    else:
+        site = site.index_select(0, idx)
+        rest = rest.index_select(1, idx)
+    return site, sigma[order], np.array(qn)[order].tolist(), rest
 
 
 def optimized_svd(a, full_matrices):
