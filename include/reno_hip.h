@@ -317,6 +317,37 @@ int reno_heff_dense(int Ml, int d, int Mr, int Dl, int Dr, const void* L,
                     const void* W, const void* R, const int64_t* idx, int nidx,
                     void* out, void* stream);
 
+/* ---- Davidson vector algebra (replaces the numpy vector work of the
+ * reference's Davidson-Liu solver, lib/davidson/davidson.py:154-441, run on
+ * the DMRG local problem at gs.py:533-538).  Basis vectors are rows of
+ * length n, row i at V + i * ldv (ldv >= n), complex128; at most 64 rows.
+ * reno_dav_orth: x <- x - V^T (conj(V) x) twice against rows 0..k-1 (the
+ * projections of davidson.py:399-402 and _qr, 453-477, with one more
+ * pass), nrm2[0] (device double) = |x|^2, and x <- x / |x| when
+ * nrm2[0] >= thresh (the lindep test of davidson.py:405-411).
+ * reno_dav_dots: out[i] = sum_r conj(V[i][r]) y[r], i < k (device
+ * complex128): a column of the subspace matrix (_fill_heff_hermitian,
+ * davidson.py:54-70).
+ * reno_dav_ritz: with U (m x k, row-major complex128) the lowest k
+ * eigenvectors of the m-dimensional subspace matrix and e (device float64)
+ * their values, X[j] = sum_i U[i][j] V[i], AX[j] = sum_i U[i][j] AV[i],
+ * R[j] = AX[j] - e[j] X[j] (rows of length n, contiguous) and rnorm2[j] =
+ * |R[j]|^2 (davidson.py:331-345, _gen_x0 479-487); m <= 64, m * k <= 1024.
+ * reno_dav_precond: out = r / (hdiag - e + 1e-4) where mask (one byte per
+ * entry) is set, 0 elsewhere (gs.py:534 on the allowed entries).
+ * Sums run as fixed-order per-block partials and one fold: the results are
+ * reproducible bit for bit. */
+size_t reno_dav_workspace_bytes(int64_t n, int k);
+int reno_dav_orth(int64_t n, int k, const void* V, int64_t ldv, void* x, double thresh, void* nrm2,
+                  void* ws, size_t ws_bytes, void* stream);
+int reno_dav_dots(int64_t n, int k, const void* V, int64_t ldv, const void* y, void* out, void* ws,
+                  size_t ws_bytes, void* stream);
+int reno_dav_ritz(int64_t n, int m, int k, const void* V, const void* AV, int64_t ldv, const void* U,
+                  const void* e, void* X, void* AX, void* R, void* rnorm2, void* ws, size_t ws_bytes,
+                  void* stream);
+int reno_dav_precond(int64_t n, const void* r, const void* hdiag, const void* mask, double e, void* out,
+                     void* stream);
+
 /* ---- tensordot (replaces matrix.py:210-211 for the 2-D contractions of
  * the sweep: absorbing the evolved bond into the neighbouring site) ------
  * C (M x N) = A (M x K) . B (K x N), all row-major complex128. */

@@ -12,8 +12,9 @@ Device parts: environments (native.env_update), the effective-Hamiltonian
 matvec (native.hop_apply; two-site problems merge the two MPO sites into one
 of physical dimension d0*d1), its diagonal and, for sites with fewer than
 1000 allowed coefficients, its dense form on the allowed entries
-(native.heff_diag / native.heff_dense), the Davidson vectors, and the
-site updates.  The host keeps the quantum-number bookkeeping, the Davidson
+(native.heff_diag / native.heff_dense), the Davidson vector algebra
+(native.dav_*: orthogonalisation, subspace-matrix columns, Ritz vectors and
+residuals, preconditioner), and the site updates.  The host keeps the quantum-number bookkeeping, the Davidson
 subspace matrix (at most a few dozen rows), the basis selection, and - as
 the reference does (gs.py:397-398, mp.py:693) - the small dense
 eigenproblems and the blocked SVD / density-matrix eigendecompositions of
@@ -56,66 +57,83 @@ def construct_mps_mpo(model, mmax, nexciton, offset=Quantity(0)):
 # Davidson (Davidson-Liu with the diagonal preconditioner), device vectors
 
 
-def davidson(aop, x0: List[torch.Tensor], precond, tol=1e-12, max_cycle=100, max_space=12,
-             lindep=1e-14, nroots=1):
+def davidson(aop, x0: List[torch.Tensor], hdiag: torch.Tensor, mask: torch.Tensor, tol=1e-12, max_cycle=100,
+             max_space=12, lindep=1e-14, nroots=1):
     """Lowest ``nroots`` eigenpairs of the Hermitian operator ``aop`` from the
-    guesses ``x0`` (flat device vectors).  The subspace grows by the
-    preconditioned residuals of the unconverged roots and restarts from the
-    Ritz vectors when it would exceed max_space + 3 (nroots - 1); a root is
-    converged when its Ritz value moved by less than ``tol`` and its residual
-    norm is below sqrt(tol) (the criteria and defaults of the reference's
-    solver, lib/davidson/davidson.py:154-441 and gs.py:536-538).
+    guesses ``x0`` (flat device vectors), Davidson-Liu with the diagonal
+    preconditioner r / (hdiag - e + 1e-4) on the entries where ``mask`` is
+    set (reference lib/davidson/davidson.py:154-441 with gs.py:533-538).
+    The subspace grows by the preconditioned residuals of the unconverged
+    roots and restarts from the Ritz vectors when it would exceed
+    max_space + 3 (nroots - 1); a root is converged when its Ritz value moved
+    by less than ``tol`` and its residual norm is below sqrt(tol) (the
+    reference's criteria and defaults).
+
+    The basis V and its images AV stay on the device as rows of two
+    preallocated buffers; the orthogonalisation, the subspace-matrix columns,
+    the Ritz vectors with their residuals and the preconditioner are the
+    native Davidson kernels (reno_dav_*).  The host keeps the subspace matrix
+    (one new column per added vector, filled Hermitian as the reference's
+    _fill_heff_hermitian does) and its eigen-decomposition.
     Returns (e, x): e a float (nroots == 1) or an array, x a list of vectors."""
     toloose = np.sqrt(tol)
     max_space = max_space + (nroots - 1) * 3
-    V: List[torch.Tensor] = []
-    AV: List[torch.Tensor] = []
+    n = x0[0].numel()
+    dev = x0[0].device
+    cap = max_space + nroots + len(x0) + 1
+    if cap > 64:
+        raise ValueError("Davidson subspace above 64 vectors")
+    V = torch.empty((cap, n), dtype=torch.complex128, device=dev)
+    AV = torch.empty_like(V)
+    h = np.zeros((cap, cap), dtype=np.complex128)
+    hdiag = hdiag.reshape(-1).to(torch.float64).contiguous()
+    mask = mask.reshape(-1).contiguous()
+    cnt = 0
 
-    def orth_add(xs):
+    def column(j):
+        col = native.dav_dots(V[:j + 1], AV[j]).cpu().numpy()
+        h[:j + 1, j] = col
+        h[j, :j] = col[:j].conj()
+        h[j, j] = col[j].real
+
+    def add(xs):
+        nonlocal cnt
         added = 0
         for x in xs:
-            x = x.clone()
-            for _ in range(2):  # two Gram-Schmidt passes
-                if V:
-                    Vt = torch.stack(V)
-                    x = x - Vt.T @ (Vt.conj() @ x)
-            nrm = torch.linalg.vector_norm(x).item()
-            if nrm * nrm < lindep:
+            x = x.reshape(-1).to(torch.complex128).clone()
+            if native.dav_orth(V[:cnt], x, lindep) < lindep:
                 continue
-            x = x / nrm
-            V.append(x)
-            AV.append(aop(x))
+            V[cnt] = x
+            AV[cnt] = aop(x).reshape(-1)
+            column(cnt)
+            cnt += 1
             added += 1
         return added
 
-    orth_add(x0)
-    if not V:
+    add(x0)
+    if cnt == 0:
         raise RuntimeError("no linearly independent Davidson guess")
     e_prev = None
     e = X = None
     for _ in range(max_cycle):
-        Vt = torch.stack(V)
-        AVt = torch.stack(AV)
-        h = (Vt.conj() @ AVt.T).cpu().numpy()
-        h = 0.5 * (h + h.conj().T)
-        w, u = scipy.linalg.eigh(h)
+        w, u = scipy.linalg.eigh(h[:cnt, :cnt])
         k = min(nroots, len(w))
         e = w[:k]
-        uc = torch.from_numpy(np.ascontiguousarray(u[:, :k])).to(Vt.device).to(Vt.dtype)
-        X = (uc.T @ Vt)              # (k, n) Ritz vectors
-        AX = (uc.T @ AVt)
-        Rv = AX - torch.from_numpy(e).to(Vt.device).to(Vt.dtype)[:, None] * X
-        rnorm = torch.linalg.vector_norm(Rv, dim=1).cpu().numpy()
+        X, AX, R, rn2 = native.dav_ritz(V[:cnt], AV[:cnt], u[:, :k], e)
+        rnorm = np.sqrt(np.maximum(rn2, 0.0))
         de = np.full(k, np.inf) if e_prev is None or len(e_prev) != k else np.abs(e - e_prev)
         conv = (de < tol) & (rnorm < toloose)
         if conv.all():
             break
         e_prev = e
-        new = [precond(Rv[i], e[i]) for i in range(k) if not conv[i]]
-        if len(V) + len(new) > max_space:
-            V[:] = [X[i] for i in range(k)]
-            AV[:] = [AX[i] for i in range(k)]
-        if orth_add(new) == 0:
+        new = [native.dav_precond(R[i], hdiag, mask, e[i]) for i in range(k) if not conv[i]]
+        if cnt + len(new) > max_space:
+            V[:k] = X
+            AV[:k] = AX
+            cnt = k
+            for j in range(k):
+                column(j)
+        if add(new) == 0:
             break
     xs = [X[i] for i in range(X.shape[0])]
     if nroots == 1:
@@ -172,7 +190,7 @@ def _eigh_local(mps, ltensor, rtensor, W, qn_mask, guesses, nroots, inverse, use
         e = float(w[0]) if nroots == 1 else w[:k]
         return e, cs, 0
     mask_t = torch.from_numpy(flat_mask).to(dev)
-    hdiag = (native.heff_diag(ltensor, W, rtensor, shape3).reshape(-1) * inverse).real
+    hdiag = (native.heff_diag(ltensor, W, rtensor, shape3).reshape(-1) * inverse).real.contiguous()
     count = [0]
 
     def aop(x):
@@ -180,11 +198,7 @@ def _eigh_local(mps, ltensor, rtensor, W, qn_mask, guesses, nroots, inverse, use
         y = native.hop_apply(ltensor, W, rtensor, x.reshape(shape3), native.PREC_FP64).reshape(-1)
         return (y * inverse) * mask_t
 
-    def precond(r, e):
-        den = hdiag - e + 1e-4
-        return torch.where(mask_t, r / den, torch.zeros_like(r))
-
-    e, xs = davidson(aop, [g * mask_t for g in guesses], precond, max_cycle=100, nroots=nroots)
+    e, xs = davidson(aop, [g * mask_t for g in guesses], hdiag, mask_t, max_cycle=100, nroots=nroots)
     return e, [_sign_fix(x) for x in xs], count[0]
 
 

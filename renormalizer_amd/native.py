@@ -35,6 +35,7 @@ EXPORTS = (
     "reno_hop_apply", "reno_hop_apply_noalloc", "reno_env_update", "reno_env_close", "reno_expm_krylov",
     "reno_expm_krylov_async", "reno_expm_krylov_mixed_async", "reno_block_qr", "reno_heff_diag", "reno_heff_dense", "reno_zgemm",
     "reno_svd_workspace_bytes", "reno_block_svd", "reno_eigh_workspace_bytes", "reno_block_eigh",
+    "reno_dav_workspace_bytes", "reno_dav_orth", "reno_dav_dots", "reno_dav_ritz", "reno_dav_precond",
     "reno_batch_scratch_bytes", "reno_batch_site_step", "reno_batch_normalize",
     "reno_autotune", "reno_autotune_reset", "reno_autotune_entries",
     "reno_probe_mfma16", "reno_prof_enable", "reno_prof_collect", "reno_prof_kind_info",
@@ -144,6 +145,17 @@ def load_library(path: str = LIB_PATH):
         lib.reno_eigh_workspace_bytes.argtypes = [c_int, c_int, ctypes.POINTER(QnBlock)]
         lib.reno_block_eigh.restype = c_int
         lib.reno_block_eigh.argtypes = [c_int, vp, c_int, ctypes.POINTER(QnBlock), vp, vp, vp, c_size, vp]
+        c_i64 = ctypes.c_int64
+        lib.reno_dav_workspace_bytes.restype = c_size
+        lib.reno_dav_workspace_bytes.argtypes = [c_i64, c_int]
+        lib.reno_dav_orth.restype = c_int
+        lib.reno_dav_orth.argtypes = [c_i64, c_int, vp, c_i64, vp, c_double, vp, vp, c_size, vp]
+        lib.reno_dav_dots.restype = c_int
+        lib.reno_dav_dots.argtypes = [c_i64, c_int, vp, c_i64, vp, vp, vp, c_size, vp]
+        lib.reno_dav_ritz.restype = c_int
+        lib.reno_dav_ritz.argtypes = [c_i64, c_int, c_int, vp, vp, c_i64, vp, vp, vp, vp, vp, vp, vp, c_size, vp]
+        lib.reno_dav_precond.restype = c_int
+        lib.reno_dav_precond.argtypes = [c_i64, vp, vp, vp, c_double, vp, vp]
         lib.reno_env_close.restype = c_int
         lib.reno_env_close.argtypes = [c_int, c_int, c_int, c_int, vp, vp, vp, vp]
         lib.reno_zgemm.restype = c_int
@@ -683,6 +695,85 @@ def env_close(X, Y):
     prec = PREC_FP64 if X.dtype == torch.complex128 else PREC_MIXED
     out = torch.empty(G, dtype=torch.complex128, device=X.device)
     check(lib().reno_env_close(prec, F, G, H, _ptr(X), _ptr(Y), _ptr(out), _stream()), "reno_env_close")
+    return out
+
+
+def _dav_rows(V):
+    if V.dtype != torch.complex128 or V.dim() != 2 or V.stride(1) != 1 or V.shape[0] > 64:
+        raise ValueError("Davidson basis: a 2-D complex128 tensor of at most 64 contiguous rows")
+    return V.shape[0], V.stride(0)
+
+
+def _dav_vec(x, n):
+    if x.dtype != torch.complex128 or not x.is_contiguous() or x.numel() != n:
+        raise ValueError("Davidson vector: a contiguous complex128 tensor of the basis length")
+
+
+def dav_orth(V, x, thresh):
+    """Orthogonalise x (in place) against the rows of V twice, then normalise
+    it when |x|^2 >= thresh.  Returns |x|^2 before normalisation (a host
+    float: the caller's linear-dependence decision)."""
+    _require_device(V, x)
+    n = x.numel()
+    k, ldv = _dav_rows(V) if V.shape[0] else (0, n)
+    _dav_vec(x, n)
+    if k:
+        assert V.shape[1] == n, "Davidson basis length mismatch"
+    nrm2 = torch.empty(1, dtype=torch.float64, device=x.device)
+    nb = lib().reno_dav_workspace_bytes(n, k)
+    ws = workspace.get(nb, x.device)
+    check(lib().reno_dav_orth(n, k, _ptr(V) if k else None, ldv, _ptr(x), float(thresh), _ptr(nrm2), _ptr(ws),
+                              nb, _stream()), "reno_dav_orth")
+    return float(nrm2.item())
+
+
+def dav_dots(V, y):
+    """conj(V) @ y for the rows of V (device complex128 vector)."""
+    _require_device(V, y)
+    k, ldv = _dav_rows(V)
+    n = V.shape[1]
+    _dav_vec(y, n)
+    out = torch.empty(k, dtype=torch.complex128, device=y.device)
+    nb = lib().reno_dav_workspace_bytes(n, k)
+    ws = workspace.get(nb, y.device)
+    check(lib().reno_dav_dots(n, k, _ptr(V), ldv, _ptr(y), _ptr(out), _ptr(ws), nb, _stream()), "reno_dav_dots")
+    return out
+
+
+def dav_ritz(V, AV, U, e):
+    """Ritz vectors X = U^T V, AX = U^T AV, residuals R = AX - e X (k x n
+    each) and |R_j|^2 (host array) for U (m x k) and e (k) from the
+    subspace eigenproblem."""
+    _require_device(V, AV)
+    m, ldv = _dav_rows(V)
+    m2, ldv2 = _dav_rows(AV)
+    assert m == m2 and ldv == ldv2 and V.shape[1] == AV.shape[1], "V / AV mismatch"
+    n = V.shape[1]
+    k = U.shape[1]
+    dev = V.device
+    Ud = torch.from_numpy(np.ascontiguousarray(U, dtype=np.complex128)).to(dev)
+    ed = torch.from_numpy(np.ascontiguousarray(e, dtype=np.float64)).to(dev)
+    X = torch.empty((k, n), dtype=torch.complex128, device=dev)
+    AX = torch.empty_like(X)
+    R = torch.empty_like(X)
+    rn2 = torch.empty(k, dtype=torch.float64, device=dev)
+    nb = lib().reno_dav_workspace_bytes(n, k)
+    ws = workspace.get(nb, dev)
+    check(lib().reno_dav_ritz(n, m, k, _ptr(V), _ptr(AV), ldv, _ptr(Ud), _ptr(ed), _ptr(X), _ptr(AX), _ptr(R),
+                              _ptr(rn2), _ptr(ws), nb, _stream()), "reno_dav_ritz")
+    return X, AX, R, rn2.cpu().numpy()
+
+
+def dav_precond(r, hdiag, mask, e):
+    """r / (hdiag - e + 1e-4) on the entries where ``mask`` (bool) is set."""
+    _require_device(r, hdiag, mask)
+    n = r.numel()
+    _dav_vec(r, n)
+    if hdiag.dtype != torch.float64 or hdiag.numel() != n or mask.dtype != torch.bool or mask.numel() != n:
+        raise ValueError("preconditioner: float64 diagonal and bool mask of the vector length")
+    out = torch.empty_like(r)
+    check(lib().reno_dav_precond(n, _ptr(r), _ptr(hdiag.contiguous()), _ptr(mask.contiguous()), float(e),
+                                 _ptr(out), _stream()), "reno_dav_precond")
     return out
 
 
