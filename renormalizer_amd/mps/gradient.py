@@ -24,7 +24,9 @@ steps approximate, not the discrete projector-splitting map itself (whose
 derivative through the gauge QR of nearly rank-deficient site tensors is
 ill-conditioned).  At full bond dimension the TDVP steps are exact and the
 gradient equals the exact one up to the quadrature error, O(dt^(2 nodes)).
-The reference computes no gradients (the C5 workload of the north-star
+``tdvp_observable`` exposes the same computation as a torch.autograd.Function
+(forward trajectory in ``forward``, costate pass in ``backward``).  The
+reference computes no gradients (the C5 workload of the north-star
 configuration asks for them)."""
 from typing import List, Sequence, Tuple
 
@@ -104,21 +106,11 @@ def product_overlaps(bra: Mps, ket: Mps, common: SiteOp, ops: Sequence[SiteOp]) 
     return np.array(out)
 
 
-def observable_gradient(mps: Mps, mpo: Mpo, dt: float, nsteps: int, obs: SiteOp, common: SiteOp,
-                        couplings: List[SiteOp], nodes: int = 3, checkpoint: int = 1, log=None):
-    """(<O>(T), [d<O>(T)/dc_k]) for T = nsteps * dt, the single-site
-    observable O = ``obs`` and the perturbations V_k = common (x)
-    couplings[k] (single-site operators; the spin-boson couplings
-    sigma_z x_k, whose coefficients are c_k = -omega_k^2 dis_k, reference
-    model/model.py:436).  ``mps`` is the device initial state (not
-    modified), ``mpo`` the Hamiltonian; the forward states are kept every
-    ``checkpoint`` steps; ``log`` (a callable taking a string) hears of
-    the progress once per checkpoint segment."""
-    if nodes not in _GAUSS:
-        raise ValueError(f"nodes must be one of {sorted(_GAUSS)}")
+def forward_states(mps: Mps, mpo: Mpo, dt: float, nsteps: int, checkpoint: int = 1):
+    """The forward TDVP-PS trajectory from ``mps`` (not modified): psi(T) and
+    the states kept every ``checkpoint`` steps (and at T), keyed by step."""
     if checkpoint < 1:
         raise ValueError("checkpoint stride must be positive")
-    s_nodes, w_nodes = _GAUSS[nodes]
     psi = mps.copy()
     if not psi.on_device:
         psi.to_device()
@@ -127,8 +119,19 @@ def observable_gradient(mps: Mps, mpo: Mpo, dt: float, nsteps: int, obs: SiteOp,
         psi = psi.evolve(mpo, dt)
         if (n + 1) % checkpoint == 0 or n + 1 == nsteps:
             saved[n + 1] = psi.copy()
-    value = product_overlaps(psi, psi, obs, [])[-1].real
-    lam = _apply_site_op(psi, obs)
+    return psi, saved
+
+
+def costate_gradient(psi_t: Mps, saved, mpo: Mpo, dt: float, nsteps: int, obs: SiteOp, common: SiteOp,
+                     couplings: List[SiteOp], nodes: int = 3, checkpoint: int = 1, log=None) -> np.ndarray:
+    """The backward pass: [d<O>(T)/dc_k] from psi(T) and the checkpoints of
+    ``forward_states`` (the same ``checkpoint`` stride), the costate lambda
+    running back from O psi(T) and the states between checkpoints
+    recomputed segment by segment."""
+    if nodes not in _GAUSS:
+        raise ValueError(f"nodes must be one of {sorted(_GAUSS)}")
+    s_nodes, w_nodes = _GAUSS[nodes]
+    lam = _apply_site_op(psi_t, obs)
     grad = np.zeros(len(couplings))
     # backward over the checkpoint segments, each recomputed from its start
     for seg_start in reversed(range(0, nsteps, checkpoint)):
@@ -148,7 +151,71 @@ def observable_gradient(mps: Mps, mpo: Mpo, dt: float, nsteps: int, obs: SiteOp,
             lam = lam.evolve(mpo, -dt)
         if log is not None:
             log(f"adjoint: steps {seg_start}..{seg_end - 1} of {nsteps} done")
-    return value, grad
+    return grad
+
+
+def observable_gradient(mps: Mps, mpo: Mpo, dt: float, nsteps: int, obs: SiteOp, common: SiteOp,
+                        couplings: List[SiteOp], nodes: int = 3, checkpoint: int = 1, log=None):
+    """(<O>(T), [d<O>(T)/dc_k]) for T = nsteps * dt, the single-site
+    observable O = ``obs`` and the perturbations V_k = common (x)
+    couplings[k] (single-site operators; the spin-boson couplings
+    sigma_z x_k, whose coefficients are c_k = -omega_k^2 dis_k, reference
+    model/model.py:436).  ``mps`` is the device initial state (not
+    modified), ``mpo`` the Hamiltonian; the forward states are kept every
+    ``checkpoint`` steps; ``log`` (a callable taking a string) hears of
+    the progress once per checkpoint segment.  The propagation runs in the
+    precision of ``mps.evolve_config``."""
+    if nodes not in _GAUSS:
+        raise ValueError(f"nodes must be one of {sorted(_GAUSS)}")
+    psi, saved = forward_states(mps, mpo, dt, nsteps, checkpoint)
+    value = product_overlaps(psi, psi, obs, [])[-1].real
+    return value, costate_gradient(psi, saved, mpo, dt, nsteps, obs, common, couplings, nodes, checkpoint, log)
+
+
+class TdvpObservable(torch.autograd.Function):
+    """<O>(T) of the TDVP-PS propagation as a differentiable function of the
+    coupling constants c (a 1-D float64 tensor): ``forward`` builds H(c)
+    through ``mpo_of`` and runs the forward trajectory, keeping the
+    checkpoints; ``backward`` runs the costate pass (costate_gradient) and
+    returns grad_output * d<O>(T)/dc."""
+
+    @staticmethod
+    def forward(ctx, c, mpo_of, mps, dt, nsteps, obs, common, couplings, nodes, checkpoint):
+        mpo = mpo_of(c.detach().cpu().numpy())
+        psi, saved = forward_states(mps, mpo, dt, nsteps, checkpoint)
+        ctx.state = (psi, saved, mpo, dt, nsteps, obs, common, couplings, nodes, checkpoint)
+        return torch.tensor(product_overlaps(psi, psi, obs, [])[-1].real, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        psi, saved, mpo, dt, nsteps, obs, common, couplings, nodes, checkpoint = ctx.state
+        ctx.state = None
+        g = costate_gradient(psi, saved, mpo, dt, nsteps, obs, common, couplings, nodes, checkpoint)
+        return (grad_output * torch.from_numpy(g),) + (None,) * 9
+
+
+def tdvp_observable(c: torch.Tensor, mpo_of, mps: Mps, dt: float, nsteps: int, obs: SiteOp, common: SiteOp,
+                    couplings: List[SiteOp], nodes: int = 3, checkpoint: int = 1) -> torch.Tensor:
+    """<O>(T) with autograd support in the couplings ``c``: H(c) = mpo_of(c)
+    must equal H_0 + sum_k c_k common (x) couplings[k] (for the spin-boson
+    model, ``spin_boson_mpo_of``)."""
+    return TdvpObservable.apply(c, mpo_of, mps, dt, nsteps, obs, common, couplings, nodes, checkpoint)
+
+
+def spin_boson_mpo_of(model):
+    """c -> the Mpo of ``model`` (a SpinBosonModel) with the couplings
+    c_k = -omega_k^2 dis_k of sigma_z x_k replaced by c (reference
+    model/model.py:436); spin splitting, tunnelling, frequencies and levels
+    kept."""
+    from renormalizer_amd.model import Phonon, SpinBosonModel
+    from renormalizer_amd.utils import Quantity
+
+    def build(c):
+        phs = [Phonon.simple_phonon(Quantity(ph.omega[0]), Quantity(-float(ck) / ph.omega[1] ** 2), ph.n_phys_dim)
+               for ph, ck in zip(model.ph_list, c)]
+        return Mpo(SpinBosonModel(Quantity(model.epsilon), Quantity(model.delta), phs))
+
+    return build
 
 
 def spin_boson_couplings(model) -> List[SiteOp]:

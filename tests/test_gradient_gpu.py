@@ -7,13 +7,19 @@ sigma_z x_k, reference model/model.py:436) on the device.
   Kronecker Hamiltonian, oracle.dense) on a spin + 3 modes model.
 * At the C5 shape (spin + 32 modes x 8 levels, M = 64), over 10 steps,
   against central finite differences (h = 1e-5) of the device TDVP-PS
-  trajectory itself for a few modes, with the difference reported."""
+  trajectory itself for a few modes, with the difference reported.
+* The torch.autograd interface (tdvp_observable) gives the adjoint's value
+  and gradient.
+* The propagation in mixed precision (FP32 matrix-core tail of each
+  Lanczos space) gives the FP64 gradient at the C5 shape to 1e-6 of its
+  largest component."""
 import numpy as np
 import pytest
 
 from oracle import dense
 from renormalizer_amd import EvolveConfig, EvolveMethod, Mpo, Phonon, Quantity, SpinBosonModel
-from renormalizer_amd.mps.gradient import observable_gradient, spin_boson_couplings, spin_boson_sigma_z
+from renormalizer_amd.mps.gradient import (observable_gradient, spin_boson_couplings, spin_boson_mpo_of,
+                                           spin_boson_sigma_z, tdvp_observable)
 from renormalizer_amd.sbm import SpectralDensityFunction
 from tests import helpers
 
@@ -103,3 +109,49 @@ def test_gradient_c5_shape(dev):
         print(f"[gradient C5] mode {k}: adjoint {grad[k]:.9f} fd {fd:.9f} |diff| {diffs[-1]:.2e}", flush=True)
     assert np.all(np.isfinite(grad)) and len(grad) == 32
     assert max(diffs) < 1e-4 * max(1.0, np.abs(grad).max())
+
+
+def test_autograd_matches_adjoint(dev):
+    """tdvp_observable under torch.autograd: the value and c.grad equal
+    observable_gradient's on the same state and Hamiltonian."""
+    import torch
+    print("[gradient autograd sbm 3 modes x 4 levels, 10 steps]", flush=True)
+    nlev, dt, nstep = 4, 0.05, 10
+    omegas, dis = _modes(3, 0.4, 5.0)
+    model = _model(omegas, dis, nlev)
+    mps, mpo = helpers.sbm_initial(model, 64)
+    mps.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps)
+    mps.to_device()
+    sz_op, cpl = spin_boson_sigma_z(model), spin_boson_couplings(model)
+    sz, grad = observable_gradient(mps, mpo, dt, nstep, sz_op, sz_op, cpl, checkpoint=3)
+    c = torch.tensor([-ph.omega[1] ** 2 * ph.dis[1] for ph in model.ph_list], dtype=torch.float64,
+                     requires_grad=True)
+    val = tdvp_observable(c, spin_boson_mpo_of(model), mps, dt, nstep, sz_op, sz_op, cpl, checkpoint=3)
+    val.backward()
+    v = val.detach().item()
+    print(f"[gradient autograd] value {v:.12f} grad {c.grad.numpy()}", flush=True)
+    assert abs(v - sz) < 1e-13
+    np.testing.assert_allclose(c.grad.numpy(), grad, rtol=1e-12, atol=1e-14)
+
+
+def test_gradient_mixed_precision(dev):
+    """C5 shape, 10 steps: the gradient with the mixed-precision propagation
+    (complex64 matrix-core tail) against the FP64 one."""
+    print("[gradient C5 shape, mixed vs FP64, 10 steps]", flush=True)
+    nlev, dt, nstep = 8, 0.05, 10
+    omegas, dis = _modes(32, 0.4, 5.0)
+    model = _model(omegas, dis, nlev)
+    mps, mpo = helpers.sbm_initial(model, 64)
+    sz_op, cpl = spin_boson_sigma_z(model), spin_boson_couplings(model)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        m = mps.copy()
+        m.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps, precision=prec)
+        m.to_device()
+        out[prec] = observable_gradient(m, mpo, dt, nstep, sz_op, sz_op, cpl, checkpoint=4)
+    (s64, g64), (smx, gmx) = out["fp64"], out["mixed"]
+    err = np.abs(gmx - g64).max()
+    print(f"[gradient C5 mixed] |d value| {abs(smx - s64):.2e}, max |d grad| {err:.2e}, "
+          f"max |grad| {np.abs(g64).max():.3e}", flush=True)
+    assert abs(smx - s64) < 1e-7
+    assert err <= 1e-6 * np.abs(g64).max()
