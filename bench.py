@@ -605,6 +605,32 @@ def summarize_counters(directory, top=12):
         print(f"| {name} | {gs} | {len(v)} | {sum(v) / len(v):.1f} | {min(v):.1f} | {max(v):.1f} |")
 
 
+def summarize_mfma(directory, top=12):
+    """Per kernel and grid size of a rocprofv3 --pmc run of MfmaUtil with
+    SQ_INSTS_VALU_MFMA_MOPS_F32 and _F64: over the dispatches that executed
+    matrix-core work, the mean MFMA utilisation (percent of SIMD cycles the
+    matrix cores are busy while the GPU is active, the profiler's derived
+    counter) and the MFMA FLOP per dispatch (MOPS x 512)."""
+    import collections
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    meta = {}
+    for r in _rocprof_csv(directory, "counter_collection.csv"):
+        d = r.get("Dispatch_Id") or r.get("Correlation_Id")
+        per[d][r["Counter_Name"]] += float(r["Counter_Value"])
+        meta[d] = (r["Kernel_Name"], r.get("Grid_Size", "?"))
+    groups = collections.defaultdict(list)
+    for d, c in per.items():
+        flop = 512.0 * (c.get("SQ_INSTS_VALU_MFMA_MOPS_F32", 0.0) + c.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0))
+        if flop > 0:
+            groups[meta[d]].append((c.get("MfmaUtil", float("nan")), flop))
+    print("| kernel | grid (work-items) | dispatches | mean MfmaUtil % | min % | max % | MFMA GFLOP per dispatch |")
+    print("|---|---|---|---|---|---|---|")
+    for (name, gs), v in sorted(groups.items(), key=lambda kv: -sum(f for _, f in kv[1]))[:top]:
+        u = [x for x, _ in v]
+        print(f"| {name} | {gs} | {len(v)} | {sum(u) / len(u):.1f} | {min(u):.1f} | {max(u):.1f} | "
+              f"{sum(f for _, f in v) / len(v) / 1e9:.3f} |")
+
+
 def summarize_gaps(directory, last_ms=None):
     """Busy and idle time of the device over a kernel trace: the union of
     the kernel intervals against the span from the first start to the last
@@ -695,6 +721,8 @@ def main():
                     help="device busy / idle time and kernel gaps of a rocprofv3 --kernel-trace directory")
     ap.add_argument("--gaps-last-ms", type=float, default=None,
                     help="with --summarize-gaps: only the last this many milliseconds of the trace")
+    ap.add_argument("--summarize-mfma", metavar="DIR",
+                    help="print the per-kernel MFMA utilisation table of a rocprofv3 --pmc MfmaUtil run and exit")
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="print the per-kernel table of a rocprofv3 --kernel-trace output directory")
     a = ap.parse_args()
@@ -703,6 +731,9 @@ def main():
         return
     if a.summarize_gaps:
         summarize_gaps(a.summarize_gaps, a.gaps_last_ms)
+        return
+    if a.summarize_mfma:
+        summarize_mfma(a.summarize_mfma)
         return
     if a.summarize_trace:
         summarize_trace(a.summarize_trace)
