@@ -42,6 +42,22 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a) * b
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// The row loops below are FP64 VALU-bound: their complex multiply-adds are
+// written as four fused multiply-adds each (a separate product and sum
+// would take six instructions).
+// (re, im) += conj(a) b
+__device__ __forceinline__ void acc_conj_mul(double& re, double& im, double2 a, double2 b) {
+  re = fma(a.x, b.x, fma(a.y, b.y, re));
+  im = fma(a.x, b.y, fma(-a.y, b.x, im));
+}
+// y + a b
+__device__ __forceinline__ double2 cmul_add(double2 y, double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, fma(-a.y, b.y, y.x)), fma(a.x, b.y, fma(a.y, b.x, y.y)));
+}
+// y - a b
+__device__ __forceinline__ double2 cmul_sub(double2 y, double2 a, double2 b) {
+  return make_double2(fma(-a.x, b.x, fma(a.y, b.y, y.x)), fma(-a.x, b.y, fma(-a.y, b.x, y.y)));
+}
 
 // Workgroup barrier ordering LDS only: every exchange between the threads
 // of these kernels goes through LDS, so a barrier need not wait for the
@@ -123,9 +139,7 @@ __device__ __forceinline__ void apply_prev_column(double2* __restrict__ X, int m
   for (int i = 0; i < RPT; ++i) {
 #pragma unroll
     for (int q = 0; q < kQrNB; ++q) {
-      const double2 p = cmulc(v[i][q], x[i]);
-      w[2 * q] += p.x;
-      w[2 * q + 1] += p.y;
+      acc_conj_mul(w[2 * q], w[2 * q + 1], v[i][q], x[i]);
     }
   }
   lds_reduce<2 * kQrNB>(w, sh, red);
@@ -144,7 +158,7 @@ __device__ __forceinline__ void apply_prev_column(double2* __restrict__ X, int m
     const int r = jp + tid + kPanelThreads * i;
     double2 y = x[i];
 #pragma unroll
-    for (int q = 0; q < kQrNB; ++q) y = cadd(y, cmul(v[i][q], g[q]));
+    for (int q = 0; q < kQrNB; ++q) y = cmul_add(y, v[i][q], g[q]);
     if (r < m) X[r + (int64_t)c * ld] = y;
   }
 }
@@ -185,13 +199,9 @@ __device__ __forceinline__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, i
       const int r = r0 + tid + kPanelThreads * i;
       if (i > 0 || r > j) {
         const double2 x = a[i][0];
-        vals[0] += x.x * x.x + x.y * x.y;
+        vals[0] = fma(x.x, x.x, fma(x.y, x.y, vals[0]));
 #pragma unroll
-        for (int c = 1; c < kQrNB; ++c) {
-          const double2 p = cmulc(x, a[i][c]);
-          vals[1 + 2 * c] += p.x;
-          vals[2 + 2 * c] += p.y;
-        }
+        for (int c = 1; c < kQrNB; ++c) acc_conj_mul(vals[1 + 2 * c], vals[2 + 2 * c], x, a[i][c]);
       } else if (r == j) {
 #pragma unroll
         for (int c = 0; c < kQrNB; ++c) pv[c] = a[i][c];
@@ -264,7 +274,7 @@ __device__ __forceinline__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, i
 #pragma unroll
         for (int c = 1; c < kQrNB; ++c) {
           double2 y = a[i][c];
-          if (r >= j) y = csub(y, cmul(vr, f[c]));
+          if (r >= j) y = cmul_sub(y, vr, f[c]);
           a[i][c - 1] = y;
         }
         a[i][kQrNB - 1] = (r < j || r >= m) ? make_double2(0.0, 0.0) : vr;
@@ -275,7 +285,7 @@ __device__ __forceinline__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, i
           Vall[r + (int64_t)j * m] = vr;
         }
 #pragma unroll
-        for (int c = 1; c < kQrNB; ++c) a[i][c - 1] = csub(a[i][c], cmul(vr, f[c]));
+        for (int c = 1; c < kQrNB; ++c) a[i][c - 1] = cmul_sub(a[i][c], vr, f[c]);
         a[i][kQrNB - 1] = vr;
       }
     }
