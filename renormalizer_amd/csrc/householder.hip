@@ -253,41 +253,51 @@ __device__ __forceinline__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, i
       f[c] = (refl && c > 0 && c < nc)
                  ? cmul(ct, cadd(pv[c], cmul(cs, make_double2(red[1 + 2 * c], red[2 + 2 * c]))))
                  : make_double2(0.0, 0.0);
+    // the reflector's rows and the final column first ...
+    double2 vr[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
       const int r = r0 + tid + kPanelThreads * i;
       const double2 x = a[i][0];
       if (i == 0) {
-        double2 vr = r > j ? cmul(x, scale) : x;  // v_r below the diagonal
-        double2 out = vr;                         // final column value
+        vr[0] = r > j ? cmul(x, scale) : x;  // v_r below the diagonal
+        double2 out = vr[0];                 // final column value
         if (r == j) {
           out = make_double2(beta, 0.0);
-          vr = make_double2(1.0, 0.0);
+          vr[0] = make_double2(1.0, 0.0);
         }
         if (r < j) out = x;
         if (r < m) {
           X[r + (int64_t)j * ld] = out;
-          Vall[r + (int64_t)j * m] = r < j ? make_double2(0.0, 0.0) : vr;
+          Vall[r + (int64_t)j * m] = r < j ? make_double2(0.0, 0.0) : vr[0];
         }
-        // later columns: a_c -= v f_c on rows >= j, then shift left (f_c is
-        // zero for the reflector slots); v_jj enters the last slot
-#pragma unroll
-        for (int c = 1; c < kQrNB; ++c) {
-          double2 y = a[i][c];
-          if (r >= j) y = cmul_sub(y, vr, f[c]);
-          a[i][c - 1] = y;
-        }
-        a[i][kQrNB - 1] = (r < j || r >= m) ? make_double2(0.0, 0.0) : vr;
       } else {
-        const double2 vr = cmul(x, scale);  // zero for rows past m
+        vr[i] = cmul(x, scale);  // zero for rows past m
         if (i < RPT - 1 || r < m) {
-          X[r + (int64_t)j * ld] = vr;
-          Vall[r + (int64_t)j * m] = vr;
+          X[r + (int64_t)j * ld] = vr[i];
+          Vall[r + (int64_t)j * m] = vr[i];
         }
-#pragma unroll
-        for (int c = 1; c < kQrNB; ++c) a[i][c - 1] = cmul_sub(a[i][c], vr, f[c]);
-        a[i][kQrNB - 1] = vr;
       }
+    }
+    // ... then the later columns, a_c -= v f_c on rows >= j, one slot at a
+    // time behind a wave-uniform test: the slots c >= nc hold reflectors
+    // (f_c = 0) and are skipped, about half of the update work
+    const bool upd0 = r0 + tid >= j;
+#pragma unroll
+    for (int c = 1; c < kQrNB; ++c) {
+      if (c < nc) {
+#pragma unroll
+        for (int i = 0; i < RPT; ++i)
+          if (i > 0 || upd0) a[i][c] = cmul_sub(a[i][c], vr[i], f[c]);
+      }
+    }
+    // shift the slots left; v_jj enters the last one
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = r0 + tid + kPanelThreads * i;
+#pragma unroll
+      for (int c = 1; c < kQrNB; ++c) a[i][c - 1] = a[i][c];
+      a[i][kQrNB - 1] = i == 0 ? ((r < j || r >= m) ? make_double2(0.0, 0.0) : vr[0]) : vr[i];
     }
   }
 }
