@@ -236,3 +236,49 @@ def test_expand_c4_device():
     bound = np.maximum(10 * np.abs(a - b), 1e-9)
     print("device - host(gesdd):", np.abs(zs - a).tolist(), "bound", bound.tolist(), flush=True)
     assert np.all(np.abs(zs - a) <= bound)
+
+
+def qn_conserved(mps):
+    """Every nonzero entry A_i[l, d, r] of every site joins bond labels with
+    qn_l + sigma_d = qn_r (left-block numbers on every bond)."""
+    m = mps.copy()
+    m.move_qnidx(len(m) - 1)
+    sites = m.numpy_sites()
+    for i, a in enumerate(sites):
+        ql = np.array(m.qn[i]).reshape(a.shape[0], -1)
+        # the last bond is stored as zeros; its left-block number is qntot
+        qr = (np.array(m.qn[i + 1]) if i + 1 < len(m) else np.array([m.qntot])).reshape(a.shape[2], -1)
+        sig = np.array(m._get_sigmaqn(i)).reshape(a.shape[1], -1)
+        big = np.abs(a) > 1e-14 * np.abs(a).max()
+        for l, dd, r in zip(*np.nonzero(big)):
+            if not np.array_equal(ql[l] + sig[dd], qr[r]):
+                return False
+    return True
+
+
+def test_expand_holstein_matches_host(target):
+    """A quantum-number-conserving expansion (one electron on a 4-site
+    Holstein chain, several sectors per bond, include_ex=True): same bond
+    dimensions and the same state as the host path, with bond quantum
+    numbers that the site tensors conserve.  The labels themselves are not
+    compared: where the fixed bond dimension exceeds a bond's rank, the
+    states kept beyond it have zero weight and their sectors (like the
+    order of degenerate singular values) are set by rounding."""
+    model = helpers.holstein_model(4, 3)
+    mpo = Mpo(model)
+
+    def build(dev):
+        gs = Mps.ground_state(model, False)
+        s = Mpo.onsite(model, r"a^\dagger", dof_set={1}) @ gs
+        s.compress_config = CompressConfig(CompressCriteria.fixed, max_bonddim=8)
+        s.evolve_config = EvolveConfig(EvolveMethod.tdvp_ps)
+        if dev:
+            s.to_device(target)
+        return s.expand_bond_dimension(mpo, coef=1e-3)
+
+    h, d = build(False), build(True)
+    assert h.bond_dims == d.bond_dims
+    assert qn_conserved(h) and qn_conserved(d)
+    vh, vd = dense(h), dense(d)
+    fid = abs(np.vdot(vh, vd)) / np.linalg.norm(vh) / np.linalg.norm(vd)
+    assert fid > 1 - 1e-12, fid
