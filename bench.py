@@ -120,6 +120,8 @@ def run_mode(mps0, mpo, precision, dt, warmup, steps, profile, tau=None, autotun
         matvecs.append(sum(mps.evolve_config.local_steps))
         low.append(getattr(mps.evolve_config, "fp32_matvecs", 0))
         sz.append(sigma_z(mps))
+        if steps > 50 and (k + 1) % 25 == 0:
+            log(f"{precision}: {k + 1} of {steps} timed steps, {seconds:.1f} s")
     prof = None
     if profile:
         prof = native.prof_collect()
