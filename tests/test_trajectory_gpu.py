@@ -18,7 +18,8 @@ perturbation moves <sigma_z> by 1e-8 within ten steps), and the fixture's
 start and the device's are built by the host expansion on different
 machines, so C5's counts are compared on identical tensors by
 test_tdvp_gpu.py::test_spin_boson_c5_shape_parity instead.  Mixed mode (C1): within the metric's 1e-6 of the oracle over
-the whole trajectory."""
+the whole trajectory; C2: within 1e-6 of the device FP64 run (and of the
+oracle up to the conditioning bound) over its 200 steps."""
 import os
 
 import numpy as np
