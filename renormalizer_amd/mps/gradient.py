@@ -26,8 +26,7 @@ ill-conditioned).  At full bond dimension the TDVP steps are exact and the
 gradient equals the exact one up to the quadrature error, O(dt^(2 nodes)).
 ``tdvp_observable`` exposes the same computation as a torch.autograd.Function
 (forward trajectory in ``forward``, costate pass in ``backward``).  The
-reference computes no gradients (the C5 workload of the north-star
-configuration asks for them)."""
+reference computes no gradients."""
 from typing import List, Sequence, Tuple
 
 import numpy as np
