@@ -172,3 +172,23 @@ def test_qn_block_descriptors_keep_index_tensors_alive(monkeypatch):
     a = native._device_index(blocks[-1][1], torch.device("cpu"))
     native._device_index(np.arange(3), torch.device("cpu"))
     assert native._device_index(blocks[-1][1], torch.device("cpu")) is a
+
+
+def test_davidson_entries_reject_bad_arguments_without_launch():
+    """The Davidson vector entries refuse a basis above 64 rows, a row
+    stride shorter than the vectors, missing pointers and a too small
+    workspace before any device work."""
+    lib = native.load_library()
+    ws = 1 << 20
+    assert lib.reno_dav_workspace_bytes(1000, 8) > 0
+    # more than 64 basis vectors
+    assert lib.reno_dav_orth(1000, 65, 1, 1000, 2, 1e-14, 3, 4, ws, None) == native.RENO_EINVAL
+    # row stride shorter than the vector
+    assert lib.reno_dav_dots(1000, 4, 1, 999, 2, 3, 4, ws, None) == native.RENO_EINVAL
+    # missing output
+    assert lib.reno_dav_ritz(1000, 8, 2, 1, 2, 1000, 3, 4, 5, 6, None, 8, 9, ws, None) == native.RENO_EINVAL
+    # more roots than basis vectors
+    assert lib.reno_dav_ritz(1000, 2, 3, 1, 2, 1000, 3, 4, 5, 6, 7, 8, 9, ws, None) == native.RENO_EINVAL
+    # workspace too small
+    assert lib.reno_dav_dots(1000, 4, 1, 1000, 2, 3, 4, 16, None) == native.RENO_ENOSPACE
+    assert lib.reno_dav_precond(0, 1, 2, 3, 0.0, 4, None) == native.RENO_EINVAL
