@@ -580,6 +580,23 @@ def traffic_probe(reps=20):
     log(f"traffic probe: {reps} one-site and bond matvecs per precision at M = {M}, d = {d}, D = {D}")
 
 
+def qr_probe(reps=10, rows=2560, cols=256):
+    """Blocked QR of one random complex rows x cols block (the full-site
+    shape) through reno_block_qr, ``reps`` times (kernel-trace and counter
+    runs of the Householder kernels)."""
+    import numpy as np
+    import torch
+    from renormalizer_amd import native
+    torch.cuda.set_device(0)
+    rng = np.random.default_rng(5)
+    t = torch.from_numpy(rng.standard_normal((rows, cols)) + 1j * rng.standard_normal((rows, cols))).to("cuda:0")
+    blocks = [(np.arange(rows), np.arange(cols))]
+    for _ in range(reps):
+        native.block_qr("L", t, blocks)
+    torch.cuda.synchronize()
+    log(f"qr probe: {reps} QRs of a {rows} x {cols} block")
+
+
 def summarize_counters(directory, top=12):
     """Per-kernel table of a rocprofv3 --pmc run: dispatches, dispatches
     whose counter reads under 1 KiB (launches cancelled on the device by the
@@ -724,6 +741,8 @@ def main():
                     help="device busy / idle time and kernel gaps of a rocprofv3 --kernel-trace directory")
     ap.add_argument("--gaps-last-ms", type=float, default=None,
                     help="with --summarize-gaps: only the last this many milliseconds of the trace")
+    ap.add_argument("--qr-probe", action="store_true",
+                    help="ten blocked QRs of a 2560 x 256 block (profiler runs of the QR kernels) and exit")
     ap.add_argument("--summarize-mfma", metavar="DIR",
                     help="print the per-kernel MFMA utilisation table of a rocprofv3 --pmc MfmaUtil run and exit")
     ap.add_argument("--summarize-trace", metavar="DIR",
@@ -737,6 +756,9 @@ def main():
         return
     if a.summarize_mfma:
         summarize_mfma(a.summarize_mfma)
+        return
+    if a.qr_probe:
+        qr_probe()
         return
     if a.summarize_trace:
         summarize_trace(a.summarize_trace)
