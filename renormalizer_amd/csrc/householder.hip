@@ -163,141 +163,166 @@ __device__ __forceinline__ void apply_prev_column(double2* __restrict__ X, int m
   }
 }
 
-// The panel columns, one runtime loop iteration per column jj (the code is
-// kept small: a fully unrolled panel overflows the instruction cache).  The
-// registers hold rows r0 + tid + 512 i (r0 <= j0, rows from r0 to j0 - 1
-// lie above the panel and pass through unchanged) of the not-yet-factored
-// columns rotated so that column jj is always slot 0 (slot c = column
-// jj + c); after each step the slots shift left by one and the new
-// reflector v_jj enters slot 7, so every register index is a compile-time
-// constant and, at step jj, v_q (q < jj) sits in slot 8 - jj + q.  One
+// One column jj of the panel.  The registers hold rows r0 + tid + 512 i
+// (r0 <= j0, rows from r0 to j0 - 1 lie above the panel and pass through
+// unchanged) of the panel in a rotating slot order: column jj + c sits in
+// slot (OFF + c) mod 8 for c < nc, and reflector v_q (q < jj) in slot
+// (OFF + 8 - jj + q) mod 8, so that every register index is a compile-time
+// constant for the caller's OFF.  The column's reflector replaces it in its
+// slot, which is where the next column (OFF + 1) expects v_jj.  One
 // reduction per column gives |x|^2, the dot products
 // g_c = sum_{r>j} conj(a_rj) a_rc with the later columns, and the same sums
 // with the earlier reflectors, from which column jj of V^H V and hence
 // column jj of the compact-WY factor T (zlarft, forward columnwise) follow
-// without another pass over V.  The pivot row (alpha, a_jc and the v_q(j))
-// goes through LDS (double-buffered by column parity).  Each column is
-// written to X / Vall as soon as it is final.  Rows of chunk i >= 1 lie
+// without another pass over V.  The pivot row (alpha, a_jc and the
+// v_q(j)) goes through LDS (double-buffered by column parity).  Each column
+// is written to X / Vall as soon as it is final.  Rows of chunk i >= 1 lie
 // below every panel column (r >= r0 + 512 > j) and, for i < RPT - 1, inside
 // the matrix; rows past m hold zeros throughout.
+template <int RPT, int OFF>
+__device__ __forceinline__ void factor_column(double2 (&a)[RPT][kQrNB], int jj, int r0, int m, int ld,
+                                              int j0, int jb, double2* __restrict__ X,
+                                              double2* __restrict__ Vall, double* sh, double* red,
+                                              double2 (*piv)[kQrNB], double2* sTau,
+                                              double2 (*sT)[kQrNB]) {
+  const int tid = threadIdx.x;
+  const int j = j0 + jj;
+  const int nc = jb - jj;  // live columns
+  double2* pv = piv[jj & 1];
+  double vals[1 + 2 * kQrNB];
+#pragma unroll
+  for (int t = 0; t < 1 + 2 * kQrNB; ++t) vals[t] = 0.0;
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    const int r = r0 + tid + kPanelThreads * i;
+    if (i > 0 || r > j) {
+      const double2 x = a[i][OFF];
+      vals[0] = fma(x.x, x.x, fma(x.y, x.y, vals[0]));
+#pragma unroll
+      for (int c = 1; c < kQrNB; ++c)
+        acc_conj_mul(vals[1 + 2 * c], vals[2 + 2 * c], x, a[i][(OFF + c) & (kQrNB - 1)]);
+    } else if (r == j) {
+#pragma unroll
+      for (int c = 0; c < kQrNB; ++c) pv[c] = a[i][(OFF + c) & (kQrNB - 1)];
+    }
+  }
+  lds_reduce<1 + 2 * kQrNB>(vals, sh, red);
+  const double xnorm2 = red[0];
+  const double2 alpha = pv[0];
+  double2 tj = make_double2(0.0, 0.0);
+  double2 scale = make_double2(1.0, 0.0);
+  double beta = alpha.x;
+  if (!(xnorm2 == 0.0 && alpha.y == 0.0)) {
+    const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
+    beta = alpha.x >= 0.0 ? -nrm : nrm;
+    const double ib = 1.0 / beta;
+    tj = make_double2((beta - alpha.x) * ib, -alpha.y * ib);
+    // scale = 1 / (alpha - beta)
+    const double dr = alpha.x - beta, di = alpha.y;
+    const double iden = 1.0 / (dr * dr + di * di);
+    scale = make_double2(dr * iden, -di * iden);
+  }
+  if (tid == 0) sTau[jj] = tj;
+  // T[q][jj] = -tau_jj sum_{p=q}^{jj-1} T[q][p] S[p][jj] for q < jj, with
+  // S[p][jj] = v_p^H v_jj = conj(v_p(j)) + scale conj(G_p), G_p the sum of
+  // conj(x) v_p over the rows below j (relative slot 8 - jj + p of the
+  // reduction).  Wave 0 forms it while the other waves update their rows.
+  if (tid < jj) {
+    const int q = tid;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int p = q; p < jj; ++p) {
+      const int sl = kQrNB - jj + p;
+      const double2 gp = make_double2(red[1 + 2 * sl], -red[2 + 2 * sl]);  // conj(G_p)
+      const double2 vpj = pv[sl];
+      const double2 spj = cadd(make_double2(vpj.x, -vpj.y), cmul(scale, gp));
+      acc = cadd(acc, cmul(sT[q][p], spj));
+    }
+    const double2 v = cmul(tj, acc);
+    sT[q][jj] = make_double2(-v.x, -v.y);
+  } else if (tid == jj) {
+    sT[jj][jj] = tj;
+  }
+  const bool refl = !(tj.x == 0.0 && tj.y == 0.0);
+  // f_c = conj(tau) (a_jc + conj(scale) g_c): coefficient of v in H^H a_c.
+  // Without a reflection scale is 1 and every f_c is zero.
+  double2 f[kQrNB];
+  const double2 ct = make_double2(tj.x, -tj.y);
+  const double2 cs = make_double2(scale.x, -scale.y);
+#pragma unroll
+  for (int c = 0; c < kQrNB; ++c)
+    f[c] = (refl && c > 0 && c < nc)
+               ? cmul(ct, cadd(pv[c], cmul(cs, make_double2(red[1 + 2 * c], red[2 + 2 * c]))))
+               : make_double2(0.0, 0.0);
+  // the reflector's rows and the final column first ...
+  double2 vr[RPT];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    const int r = r0 + tid + kPanelThreads * i;
+    const double2 x = a[i][OFF];
+    if (i == 0) {
+      vr[0] = r > j ? cmul(x, scale) : x;  // v_r below the diagonal
+      double2 out = vr[0];                 // final column value
+      if (r == j) {
+        out = make_double2(beta, 0.0);
+        vr[0] = make_double2(1.0, 0.0);
+      }
+      if (r < j) out = x;
+      if (r < m) {
+        X[r + (int64_t)j * ld] = out;
+        Vall[r + (int64_t)j * m] = r < j ? make_double2(0.0, 0.0) : vr[0];
+      }
+    } else {
+      vr[i] = cmul(x, scale);  // zero for rows past m
+      if (i < RPT - 1 || r < m) {
+        X[r + (int64_t)j * ld] = vr[i];
+        Vall[r + (int64_t)j * m] = vr[i];
+      }
+    }
+  }
+  // ... then the later columns, a_c -= v f_c on rows >= j, one slot at a
+  // time behind a wave-uniform test: the slots c >= nc hold reflectors
+  // (f_c = 0) and are skipped, about half of the update work
+  const bool upd0 = r0 + tid >= j;
+#pragma unroll
+  for (int c = 1; c < kQrNB; ++c) {
+    if (c < nc) {
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+        if (i > 0 || upd0) a[i][(OFF + c) & (kQrNB - 1)] = cmul_sub(a[i][(OFF + c) & (kQrNB - 1)], vr[i], f[c]);
+    }
+  }
+  // v_jj takes the column's slot
+  {
+    const int r = r0 + tid;
+    a[0][OFF] = (r < j || r >= m) ? make_double2(0.0, 0.0) : vr[0];
+#pragma unroll
+    for (int i = 1; i < RPT; ++i) a[i][OFF] = vr[i];
+  }
+}
+
+// The panel, two columns per loop iteration with their slot offsets fixed
+// at compile time, then the slots rotate by two (one register rotation per
+// two columns instead of one per column; a fully unrolled panel overflows
+// the instruction cache, and four columns per iteration exceed the register
+// budget).
 template <int RPT>
 __device__ __forceinline__ void factor_panel(double2 (&a)[RPT][kQrNB], int r0, int m, int ld, int j0, int jb,
                              double2* __restrict__ X, double2* __restrict__ Vall, double* sh,
                              double* red, double2 (*piv)[kQrNB], double2* sTau,
                              double2 (*sT)[kQrNB]) {
-  const int tid = threadIdx.x;
 #pragma unroll 1
-  for (int jj = 0; jj < jb; ++jj) {
-    const int j = j0 + jj;
-    const int nc = jb - jj;  // live slots
-    double2* pv = piv[jj & 1];
-    double vals[1 + 2 * kQrNB];
+  for (int jj = 0; jj < jb; jj += 2) {
+    factor_column<RPT, 0>(a, jj, r0, m, ld, j0, jb, X, Vall, sh, red, piv, sTau, sT);
+    if (jj + 1 < jb) factor_column<RPT, 1>(a, jj + 1, r0, m, ld, j0, jb, X, Vall, sh, red, piv, sTau, sT);
+    if (jj + 2 < jb) {
 #pragma unroll
-    for (int t = 0; t < 1 + 2 * kQrNB; ++t) vals[t] = 0.0;
+      for (int i = 0; i < RPT; ++i) {
+        double2 t[kQrNB];
 #pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-      const int r = r0 + tid + kPanelThreads * i;
-      if (i > 0 || r > j) {
-        const double2 x = a[i][0];
-        vals[0] = fma(x.x, x.x, fma(x.y, x.y, vals[0]));
+        for (int c = 0; c < kQrNB; ++c) t[c] = a[i][(c + 2) & (kQrNB - 1)];
 #pragma unroll
-        for (int c = 1; c < kQrNB; ++c) acc_conj_mul(vals[1 + 2 * c], vals[2 + 2 * c], x, a[i][c]);
-      } else if (r == j) {
-#pragma unroll
-        for (int c = 0; c < kQrNB; ++c) pv[c] = a[i][c];
+        for (int c = 0; c < kQrNB; ++c) a[i][c] = t[c];
       }
-    }
-    lds_reduce<1 + 2 * kQrNB>(vals, sh, red);
-    const double xnorm2 = red[0];
-    const double2 alpha = pv[0];
-    double2 tj = make_double2(0.0, 0.0);
-    double2 scale = make_double2(1.0, 0.0);
-    double beta = alpha.x;
-    if (!(xnorm2 == 0.0 && alpha.y == 0.0)) {
-      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
-      beta = alpha.x >= 0.0 ? -nrm : nrm;
-      const double ib = 1.0 / beta;
-      tj = make_double2((beta - alpha.x) * ib, -alpha.y * ib);
-      // scale = 1 / (alpha - beta)
-      const double dr = alpha.x - beta, di = alpha.y;
-      const double iden = 1.0 / (dr * dr + di * di);
-      scale = make_double2(dr * iden, -di * iden);
-    }
-    if (tid == 0) sTau[jj] = tj;
-    // T[q][jj] = -tau_jj sum_{p=q}^{jj-1} T[q][p] S[p][jj] for q < jj, with
-    // S[p][jj] = v_p^H v_jj = conj(v_p(j)) + scale conj(G_p), G_p the sum of
-    // conj(x) v_p over the rows below j (slot 8 - jj + p of the reduction)
-    if (tid < jj) {
-      const int q = tid;
-      double2 acc = make_double2(0.0, 0.0);
-      for (int p = q; p < jj; ++p) {
-        const int sl = kQrNB - jj + p;
-        const double2 gp = make_double2(red[1 + 2 * sl], -red[2 + 2 * sl]);  // conj(G_p)
-        const double2 vpj = pv[sl];
-        const double2 spj = cadd(make_double2(vpj.x, -vpj.y), cmul(scale, gp));
-        acc = cadd(acc, cmul(sT[q][p], spj));
-      }
-      const double2 v = cmul(tj, acc);
-      sT[q][jj] = make_double2(-v.x, -v.y);
-    } else if (tid == jj) {
-      sT[jj][jj] = tj;
-    }
-    const bool refl = !(tj.x == 0.0 && tj.y == 0.0);
-    // f_c = conj(tau) (a_jc + conj(scale) g_c): coefficient of v in H^H a_c.
-    // Without a reflection scale is 1 and every f_c is zero.
-    double2 f[kQrNB];
-    const double2 ct = make_double2(tj.x, -tj.y);
-    const double2 cs = make_double2(scale.x, -scale.y);
-#pragma unroll
-    for (int c = 0; c < kQrNB; ++c)
-      f[c] = (refl && c > 0 && c < nc)
-                 ? cmul(ct, cadd(pv[c], cmul(cs, make_double2(red[1 + 2 * c], red[2 + 2 * c]))))
-                 : make_double2(0.0, 0.0);
-    // the reflector's rows and the final column first ...
-    double2 vr[RPT];
-#pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-      const int r = r0 + tid + kPanelThreads * i;
-      const double2 x = a[i][0];
-      if (i == 0) {
-        vr[0] = r > j ? cmul(x, scale) : x;  // v_r below the diagonal
-        double2 out = vr[0];                 // final column value
-        if (r == j) {
-          out = make_double2(beta, 0.0);
-          vr[0] = make_double2(1.0, 0.0);
-        }
-        if (r < j) out = x;
-        if (r < m) {
-          X[r + (int64_t)j * ld] = out;
-          Vall[r + (int64_t)j * m] = r < j ? make_double2(0.0, 0.0) : vr[0];
-        }
-      } else {
-        vr[i] = cmul(x, scale);  // zero for rows past m
-        if (i < RPT - 1 || r < m) {
-          X[r + (int64_t)j * ld] = vr[i];
-          Vall[r + (int64_t)j * m] = vr[i];
-        }
-      }
-    }
-    // ... then the later columns, a_c -= v f_c on rows >= j, one slot at a
-    // time behind a wave-uniform test: the slots c >= nc hold reflectors
-    // (f_c = 0) and are skipped, about half of the update work
-    const bool upd0 = r0 + tid >= j;
-#pragma unroll
-    for (int c = 1; c < kQrNB; ++c) {
-      if (c < nc) {
-#pragma unroll
-        for (int i = 0; i < RPT; ++i)
-          if (i > 0 || upd0) a[i][c] = cmul_sub(a[i][c], vr[i], f[c]);
-      }
-    }
-    // shift the slots left; v_jj enters the last one
-#pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-      const int r = r0 + tid + kPanelThreads * i;
-#pragma unroll
-      for (int c = 1; c < kQrNB; ++c) a[i][c - 1] = a[i][c];
-      a[i][kQrNB - 1] = i == 0 ? ((r < j || r >= m) ? make_double2(0.0, 0.0) : vr[0]) : vr[i];
     }
   }
 }
