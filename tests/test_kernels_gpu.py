@@ -235,7 +235,7 @@ def test_expm_krylov_bond(dev):
 @pytest.mark.parametrize("rows,cols,nblk", [(20, 8, 1), (8, 20, 1), (64, 32, 2), (30, 30, 2), (6, 3, 1), (3, 6, 1),
                                             (2560, 256, 1), (256, 2560, 1), (640, 64, 2), (1, 7, 1), (7, 1, 1),
                                             (3100, 40, 1), (40, 3100, 1), (363, 3630, 1), (3630, 363, 1),
-                                            (300, 2800, 2)])
+                                            (300, 2800, 2), (2560, 37, 1), (1100, 45, 1), (45, 1900, 1)])
 def test_block_qr(dev, system, rows, cols, nblk):
     print(f"[block_qr {system} {rows}x{cols} blocks={nblk}]", flush=True)
     rng = np.random.default_rng(rows * 31 + cols + nblk)
